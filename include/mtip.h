@@ -1,0 +1,238 @@
+/*
+ * libmtip -- C ABI of the MI355X (gfx950) orientation-matching path:
+ * type-2 3D NUFFT slicing of an autocorrelation model at rotated
+ * Ewald-sphere pixel q-vectors, slice-to-image distances with the cross
+ * term on MFMA, and per-image argmin / top-K with FP32/FP64 rescoring.
+ *
+ * Conventions
+ *  - Every pointer named as "device" is device memory on the plan's /
+ *    call's device; the library never copies between host and device
+ *    except for its own small tables.  Host pointers are marked "host".
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = the
+ *    null stream) unless stated otherwise.
+ *  - Every function returns MTIP_OK (0) or a negative MTIP_ERR_* code;
+ *    mtip_last_error() then gives a thread-local message.
+ *  - Arrays are C order.  Models are cubic M^3 float32, model index i along
+ *    an axis is frequency k = i - M/2 (finufft's CMCL mode order); axis 0
+ *    pairs with H (x), axis 2 with L (z).
+ *  - No entry point allocates device memory except mtip_plan_create /
+ *    mtip_plan_set_model (the plan's fine grid and tables); workspaces are
+ *    passed in by the caller with their size queried first.
+ */
+#ifndef MTIP_H
+#define MTIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTIP_ABI_VERSION 1
+
+enum {
+    MTIP_OK = 0,
+    MTIP_ERR_ARG = -1,          /* bad shape, size, pointer or option */
+    MTIP_ERR_DOMAIN = -2,       /* NU point outside [-3 pi, 3 pi) */
+    MTIP_ERR_HIP = -3,          /* a HIP runtime call failed */
+    MTIP_ERR_NOMEM = -4,        /* device allocation failed / workspace too small */
+    MTIP_ERR_STATE = -5,        /* call order (e.g. execute before set_model) */
+    MTIP_ERR_UNSUPPORTED = -6   /* option not implemented */
+};
+
+/* Arithmetic of the cross-term GEMM. */
+enum { MTIP_FP32 = 0, MTIP_BF16 = 1, MTIP_FP8 = 2 };
+
+int mtip_abi_version(void);
+const char *mtip_last_error(void);
+/* host out: free and total bytes of HBM on `device` (admission control). */
+int mtip_mem_info(int device, size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------------------ NUFFT */
+
+typedef struct mtip_plan mtip_plan;
+
+/*
+ * Type-2 3D NUFFT plan for a cubic M^3 real model at tolerance eps
+ * (ES kernel, upsampling 2, width ceil(-log10(eps/10))).
+ * Replaces cufinufft(2, ugrid.shape, 1, eps, isign, dtype, gpu_method=1,
+ * gpu_device_id) -- spinifel/extern/nufft_ext.py:271-280,
+ * spinifel/extern/cufinufft_ext.py:340-342 -- and the setup half of
+ * finufftpy.nufft3d2 (spinifel/extern/finufft_ext.py:88).
+ * The plan computes the real part of the transform only, which is all the
+ * reference keeps (spinifel/sequential/orientation_matching.py:241,
+ * spinifel/slicing.py:43); the sign of the exponent therefore does not
+ * matter for it.
+ */
+int mtip_plan_create(int M, double eps, int device, mtip_plan **out);
+int mtip_plan_destroy(mtip_plan *plan);
+/* host out: fine grid n, kernel width w, ES beta, padded grid edge. */
+int mtip_plan_info(const mtip_plan *plan, int *n_fine, int *width, double *beta,
+                   int *padded_edge);
+
+/*
+ * Load a model: symmetrise (Re of the transform = transform of
+ * (a(k) + a(-k)) / 2), deconvolve by the kernel's Fourier series, and
+ * evaluate the real padded fine grid by separable FP64 DFTs; the grid stays
+ * resident in the plan.  ac: device float32 M^3.
+ * Replaces the ugrid upload and the deconvolve/FFT inside
+ * plan.execute(nuvect, ugrid) (spinifel/extern/nufft_ext.py:283,
+ * cufinufft_ext.py:344) and autocorrelation.forward's `ugrid *= support`
+ * with support = 1 (spinifel/autocorrelation.py:53-54).
+ */
+int mtip_plan_set_model(mtip_plan *plan, const float *ac, void *stream);
+
+/*
+ * Re of c_j = sum_k f_k exp(-i k . (x_j, y_j, z_j)) at N arbitrary points
+ * (radians).  x, y, z: device FP64; out: device FP32 (N).  Points with a
+ * coordinate outside [-3 pi, 3 pi) (or not finite) give NaN and add 1 to
+ * *bad_count (device int32, nullable).
+ * Replaces plan.set_pts(H_, K_, L_) + plan.execute(nuvect, ugrid)
+ * (spinifel/extern/nufft_ext.py:282-283, cufinufft_ext.py:343-344) and
+ * finufftpy.nufft3d2(x, y, z, c, isign, eps, f) (finufft_ext.py:88).
+ */
+int mtip_plan_execute(mtip_plan *plan, const double *x, const double *y, const double *z,
+                      int64_t N, float *out, int *bad_count, void *stream);
+
+/*
+ * Fused rotation + type-2 slicing for N_o orientations x N_p pixels.
+ * For each orientation n (quaternion [w, x, y, z], device FP64 (N_o, 4))
+ * the library forms R(q) by skopi's formula (eval/geom.py:26-46) and its
+ * FP64 inverse (spinifel/sequential/orientation_matching.py:220), rotates
+ * the pixel q-vectors q (device FP64 (3, N_p)) as
+ * einsum("ijk,klmn->jilmn") does (:233), scales by `scale` = pi / (E *
+ * oversampling) (:234-236), and evaluates Re of the type-2 transform.
+ * out: device FP32 (N_o, ld_out), orientation-major like model_slices_new
+ * (:238-241).  Optional (nullable) outputs: out_bf16 (N_o, ld_out) bf16 bit
+ * patterns of the same values; row_sum / row_sumsq: device FP64 (N_o)
+ * sums of s and s^2 per orientation (for the std scaling :245-248 and the
+ * squared norms of the distance expansion).  bad_count as for execute.
+ * Replaces the batched loop of spinifel/sequential/orientation_matching.py:230-241
+ * and gen_model_slices (spinifel/slicing.py:10-45).
+ */
+int mtip_plan_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
+                     int64_t N_p, double scale, float *out, int64_t ld_out, uint16_t *out_bf16,
+                     double *row_sum, double *row_sumsq, int *bad_count, void *stream);
+
+/*
+ * Rotated, normalised q-points only: H/K/L (device FP64, N_o * N_p,
+ * orientation-major) = scale * inv(R(q_n)) . q_j with R from skopi's
+ * formula, as gen_nonuniform_normalized_positions computes on the host
+ * (spinifel/autocorrelation.py:290-323; orientation_matching.py:220, :233-236).
+ * N_o <= 65535 per call.
+ */
+int mtip_rotate_positions(const double *quats, int64_t N_o, const double *q, int64_t N_p,
+                          double scale, double *H, double *K, double *L, void *stream);
+
+/* ---------------------------------------------------------- row utilities */
+
+/* Per-row FP64 sum and sum of squares of a float32 (rows, cols) matrix. */
+int mtip_row_stats(const float *in, int64_t rows, int64_t cols, int64_t ld, double *row_sum,
+                   double *row_sumsq, void *stream);
+
+/*
+ * alpha = std(images) / std(slices), population std over all entries, FP64,
+ * from per-row sums (spinifel/sequential/orientation_matching.py:246).
+ * alpha: device FP64 scalar (stays on the device for mtip_match_topk).
+ */
+int mtip_scaling_ratio(const double *img_sum, const double *img_sumsq, int64_t n_img,
+                       const double *sl_sum, const double *sl_sumsq, int64_t n_sl,
+                       int64_t n_pix, double *alpha, void *stream);
+
+/*
+ * Cast rows to the GEMM operand format.  format MTIP_BF16: out is uint16
+ * bf16 (round to nearest even), scales unused (nullable).  format MTIP_FP8:
+ * out is uint8 OCP e4m3 with a per-row scale = absmax / 448 written to
+ * scales (device float32 (rows)), value = e4m3 * scale.
+ */
+int mtip_quantize_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in, int format,
+                       void *out, int64_t ld_out, float *scales, void *stream);
+
+/*
+ * Residual rows for the GEMM operands: out_r = cast(m * in_r - center) with
+ * m = *mult (or 1 / *mult when invert_mult; 1 when mult is NULL), center a
+ * device float32 (cols) vector (NULL = 0), format MTIP_FP32 / MTIP_BF16 /
+ * MTIP_FP8 (e4m3 with per-row scales absmax / 448).  sumsq (nullable):
+ * device FP64 (rows) squared norms of the values as cast.  Distances are
+ * invariant to subtracting one vector from both slices and images, so a
+ * common centre removes the orientation-independent part of the pattern
+ * (the bright low-q region) before the cast.  Columns [cols, ld_out) are 0.
+ */
+int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in,
+                       const double *mult, int invert_mult, const float *center, int format,
+                       void *out, int64_t ld_out, float *scales, double *sumsq, void *stream);
+
+/* Column means (FP64 accumulation) of a float32 (rows, cols) matrix -> float32 (cols). */
+int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, float *mean,
+                  void *stream);
+
+/* ------------------------------------------------------------- matching */
+
+/* host: bytes of workspace mtip_match_topk needs. */
+size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision);
+
+/*
+ * Per image i the K orientations with the smallest
+ *   v(n, i) = alpha^2 ||s_n||^2 - 2 alpha s_n . d_i
+ * (= squared distance minus the image's own ||d_i||^2, which does not move
+ * the argmin), cross term on MFMA in `precision` with FP32 accumulation.
+ * slices (N_o, ld_s) and images (N_i, ld_d) are float32 / bf16 / e4m3 per
+ * `precision`; slice_scale / image_scale per-row dequantisation factors
+ * (FP8 only, else nullable); slice_sumsq: device FP64 (N_o) ||s_n||^2 of the
+ * unscaled FP32 slices; alpha: device FP64 scalar.  K in {1, 2, 4, 8}.
+ * Outputs: top_idx device int32 (N_i, K) and top_val device float32
+ * (N_i, K), ascending, lowest index first on equal values (the first-
+ * occurrence rule of numpy's argmin, spinifel/sequential/nearest_neighbor.py:135,
+ * and the strict < of orientation_matching_sp.cu:549-557).
+ * N_p must be a multiple of 64 (32 for FP32); ld_s, ld_d likewise.
+ * Replaces SNM.euclidean_dist + dist.argmin (spinifel/sequential/
+ * orientation_matching.py:81-109, :179) and the KNN pair
+ * cudaEuclideanDistance / cudaHeapSort (orientation_matching_sp.cu:612, :682)
+ * without materialising the (N_o, N_i) matrix.
+ */
+int mtip_match_topk(int precision, const void *slices, int64_t ld_s, const float *slice_scale,
+                    const double *slice_sumsq, int64_t N_o, const void *images, int64_t ld_d,
+                    const float *image_scale, int64_t N_i, int64_t N_p, const double *alpha,
+                    int K, void *workspace, size_t workspace_bytes, int32_t *top_idx,
+                    float *top_val, void *stream);
+
+/*
+ * Rescore K candidates per image on FP32 slices/images:
+ *   d(n, i) = sum_p (alpha s_n[p] - d_i[p])^2
+ * once with FP32 arithmetic (d32, nullable) and once in FP64 (d64,
+ * nullable); best_idx / best_d64 (device, N_i) take the FP64 minimum
+ * (lowest orientation index on ties).  cand: device int32 (N_i, K);
+ * entries outside [0, N_o) are skipped (their d32 / d64 are NaN).  This is the distance of
+ * spinifel/sequential/nearest_neighbor.py:134 (squared, before the sqrt).
+ */
+int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
+                 int64_t N_i, int64_t N_p, const double *alpha, const int32_t *cand, int K,
+                 int32_t *best_idx, double *best_d64, float *d32, double *d64, void *stream);
+
+/*
+ * Full squared-distance matrix euDist[i * numRef + n] = ||data_i - ref_n||^2
+ * (FP32 MFMA cross term, FP64-accumulated norms), all totalPixels pixels.
+ * Replaces cudaEuclideanDistance(refImgs, dataImgs, numRefImgs,
+ * numDataImgs, totalPixels, deviceId) (orientation_matching_sp.cu:612-679)
+ * minus its host copies and its dropping of the last totalPixels % 16
+ * pixels (:110).  totalPixels must be a multiple of 32.
+ */
+int mtip_euclidean_distance(const float *ref, const float *data, int64_t numRef, int64_t numData,
+                            int64_t totalPixels, float *euDist, void *stream);
+
+/*
+ * Indices of the totalNN smallest entries of each row of euDist
+ * (numData, numRef), ascending, lowest index first on ties: idx device
+ * int32 (numData, totalNN).  totalNN in 1..8.
+ * Replaces cudaHeapSort(euDist, numDataImgs, numRefImgs, totalPixels,
+ * totalNN, deviceId) (orientation_matching_sp.cu:682-729).
+ */
+int mtip_heap_sort(const float *euDist, int64_t numData, int64_t numRef, int totalNN,
+                   int32_t *idx, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTIP_H */

@@ -1,0 +1,812 @@
+// Slice-to-image distances on gfx950: MFMA cross-term GEMM with a fused
+// distance epilogue and per-image top-K, top-K merge, FP32/FP64 rescoring,
+// and the small row utilities around them.
+//
+// Distance expansion (per image i, orientation n, scaled slices alpha s_n):
+//   ||alpha s_n - d_i||^2 = alpha^2 ||s_n||^2 - 2 alpha s_n . d_i + ||d_i||^2
+// The last term is constant per image, so ranking uses
+//   v(n, i) = alpha^2 ||s_n||^2 - 2 alpha (s_n . d_i)
+// and the rescoring kernel recomputes the full distance for candidates.
+//
+// GEMM tiling: a 256-thread workgroup owns a 128 (orientations) x 128
+// (images) tile; waves form a 2 x 2 grid of 64 x 64 sub-tiles, each a 2 x 2
+// set of 32 x 32 MFMA blocks.  K is staged 128 bytes per row at a time
+// (32 fp32 / 64 bf16 / 128 e4m3) through LDS with a register prefetch of the
+// next stage.  Operand fragments: the lower half-wave (lanes 0-31) feeds the
+// first half of each MFMA's k-range and the upper half the second; A and B
+// use the same k assignment, so every pixel is paired with itself whatever
+// order the hardware sums in.
+#include <math.h>
+
+#include "mtip_internal.h"
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
+
+constexpr int BM = 128, BN = 128, BKB = 128;
+constexpr int LDS_STRIDE = BKB + 16;      // bytes per staged row (conflict-free b128 reads)
+constexpr uint64_t KEY_NONE = ~0ull;
+
+enum { OUT_TOPK = 0, OUT_FULL = 1 };
+
+template <int PREC> struct Prec;
+template <> struct Prec<MTIP_FP32> { static constexpr int bytes = 4; };
+template <> struct Prec<MTIP_BF16> { static constexpr int bytes = 2; };
+template <> struct Prec<MTIP_FP8> { static constexpr int bytes = 1; };
+
+__device__ __forceinline__ uint4 load16(const uint8_t *p) { return *(const uint4 *)p; }
+
+// One MFMA step on a 32 x 32 block.  `off` is the byte offset of this
+// step inside the staged 128-byte row.
+template <int PREC>
+__device__ __forceinline__ void mfma_stage(const uint8_t *sA_row0, const uint8_t *sA_row1,
+                                           const uint8_t *sB_row0, const uint8_t *sB_row1,
+                                           int half, floatx16 (&acc)[2][2])
+{
+    if constexpr (PREC == MTIP_FP32) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int off = g * 32 + half * 16;
+            floatx4 a0 = *(const floatx4 *)(sA_row0 + off);
+            floatx4 a1 = *(const floatx4 *)(sA_row1 + off);
+            floatx4 b0 = *(const floatx4 *)(sB_row0 + off);
+            floatx4 b1 = *(const floatx4 *)(sB_row1 + off);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[t], b0[t], acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[t], b1[t], acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[t], b0[t], acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[t], b1[t], acc[1][1], 0, 0, 0);
+            }
+        }
+    } else if constexpr (PREC == MTIP_BF16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int off = s * 32 + half * 16;
+            bf16x8 a0 = *(const bf16x8 *)(sA_row0 + off);
+            bf16x8 a1 = *(const bf16x8 *)(sA_row1 + off);
+            bf16x8 b0 = *(const bf16x8 *)(sB_row0 + off);
+            bf16x8 b1 = *(const bf16x8 *)(sB_row1 + off);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int off = s * 64 + half * 32;
+            intx8 a0 = *(const intx8 *)(sA_row0 + off);
+            intx8 a1 = *(const intx8 *)(sA_row1 + off);
+            intx8 b0 = *(const intx8 *)(sB_row0 + off);
+            intx8 b1 = *(const intx8 *)(sB_row1 + off);
+            // format 0 = e4m3 for both operands; E8M0 block scales 127 = x1
+            acc[0][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, b0, acc[0][0], 0, 0, 0, 127, 0, 127);
+            acc[0][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, b1, acc[0][1], 0, 0, 0, 127, 0, 127);
+            acc[1][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b0, acc[1][0], 0, 0, 0, 127, 0, 127);
+            acc[1][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b1, acc[1][1], 0, 0, 0, 127, 0, 127);
+        }
+    }
+}
+
+// A: rows = orientations (slices), B: rows = images.  kbytes = N_p * bytes.
+template <int PREC, int K, int OUT>
+__global__ __launch_bounds__(256) void k_gemm(const uint8_t *__restrict__ A, int64_t lda,
+                                              int64_t NA, const uint8_t *__restrict__ B,
+                                              int64_t ldb, int64_t NB, int64_t kbytes,
+                                              const float *__restrict__ scaleA,
+                                              const float *__restrict__ scaleB,
+                                              const double *__restrict__ sumsqA,
+                                              const double *__restrict__ alpha, int n_tiles_a,
+                                              uint64_t *__restrict__ part, float *__restrict__ full,
+                                              int64_t ldf)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t smem[(BM + BN) * LDS_STRIDE];
+    __shared__ float s_rowc[BM];     // TOPK: alpha^2 ||s||^2 ; FULL: ||a||^2
+    __shared__ float s_colc[BN];     // FULL: ||b||^2
+    __shared__ float s_sa[BM], s_sb[BN];
+    uint8_t *sA = smem;
+    uint8_t *sB = smem + BM * LDS_STRIDE;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int ta = blockIdx.x % n_tiles_a, tb = blockIdx.x / n_tiles_a;
+    const int64_t a0 = (int64_t)ta * BM, b0 = (int64_t)tb * BN;
+
+    if (tid < BM) {
+        int64_t r = a0 + tid;
+        float sc = 1.0f;
+        if (PREC == MTIP_FP8 && r < NA) sc = scaleA[r];
+        s_sa[tid] = sc;
+        if (OUT == OUT_TOPK) {
+            double al = *alpha;
+            s_rowc[tid] = r < NA ? (float)(al * al * sumsqA[r]) : 0.0f;
+        }
+    } else {
+        int64_t r = b0 + (tid - BM);
+        float sc = 1.0f;
+        if (PREC == MTIP_FP8 && r < NB) sc = scaleB[r];
+        s_sb[tid - BM] = sc;
+    }
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    // staging: 1024 16-byte chunks per operand tile, 4 per thread
+    uint4 ra[4], rb[4];
+    auto load_stage = [&](int64_t kb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int c = tid + 256 * i;
+            int r = c >> 3, col = (c & 7) * 16;
+            int64_t gk = kb + col;
+            int64_t ga = a0 + r, gb = b0 + r;
+            ra[i] = (ga < NA && gk < kbytes) ? load16(A + ga * lda + gk) : make_uint4(0, 0, 0, 0);
+            rb[i] = (gb < NB && gk < kbytes) ? load16(B + gb * ldb + gk) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    double nrm = 0.0;    // FULL: FP64 squared norm of this thread's staged row
+
+    load_stage(0);
+    for (int64_t kb = 0; kb < kbytes; kb += BKB) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int c = tid + 256 * i;
+            int r = c >> 3, col = (c & 7) * 16;
+            *(uint4 *)(sA + r * LDS_STRIDE + col) = ra[i];
+            *(uint4 *)(sB + r * LDS_STRIDE + col) = rb[i];
+        }
+        __syncthreads();
+        if (kb + BKB < kbytes) load_stage(kb + BKB);
+        if (OUT == OUT_FULL) {
+            const float *row = (const float *)((tid < BM ? sA + tid * LDS_STRIDE
+                                                         : sB + (tid - BM) * LDS_STRIDE));
+#pragma unroll
+            for (int e = 0; e < BKB / 4; ++e) nrm += (double)row[e] * (double)row[e];
+        }
+        const uint8_t *ar0 = sA + (wm * 64 + l32) * LDS_STRIDE;
+        const uint8_t *ar1 = sA + (wm * 64 + 32 + l32) * LDS_STRIDE;
+        const uint8_t *br0 = sB + (wn * 64 + l32) * LDS_STRIDE;
+        const uint8_t *br1 = sB + (wn * 64 + 32 + l32) * LDS_STRIDE;
+        mfma_stage<PREC>(ar0, ar1, br0, br1, half, acc);
+    }
+
+    if constexpr (OUT == OUT_FULL) {
+        __syncthreads();
+        if (tid < BM) s_rowc[tid] = (float)nrm;
+        else s_colc[tid - BM] = (float)nrm;
+        __syncthreads();
+        const bool vec_ok = (ldf % 4) == 0;
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) {
+            const int j = wn * 64 + bn * 32 + l32;
+            const int64_t gi = b0 + j;
+            if (gi >= NB) continue;
+            const float nb = s_colc[j];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int m = wm * 64 + bm * 32 + 8 * g + 4 * half;
+                    const int64_t go = a0 + m;
+                    float o[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        o[t] = fmaxf(s_rowc[m + t] + nb - 2.0f * acc[bm][bn][4 * g + t], 0.0f);
+                    float *dst = full + gi * ldf + go;
+                    if (vec_ok && go + 3 < NA) {
+                        *(floatx4 *)dst = floatx4{o[0], o[1], o[2], o[3]};
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (go + t < NA) dst[t] = o[t];
+                    }
+                }
+            }
+        }
+    } else {
+        const float two_a = (float)(2.0 * (*alpha));
+        uint64_t top[2][K];
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) {
+#pragma unroll
+            for (int r = 0; r < K; ++r) top[bn][r] = KEY_NONE;
+            const int j = wn * 64 + bn * 32 + l32;
+            const float sb = s_sb[j];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int m = wm * 64 + bm * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+                    const int64_t go = a0 + m;
+                    if (go < NA) {
+                        float c = acc[bm][bn][v];
+                        if (PREC == MTIP_FP8) c *= s_sa[m] * sb;
+                        float val = fmaf(-two_a, c, s_rowc[m]);
+                        mtip::topk_insert<K>(top[bn], mtip::make_key(val, (uint32_t)go));
+                    }
+                }
+            }
+            uint64_t other[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[bn][r], 32);
+            mtip::topk_merge<K>(top[bn], other);
+        }
+        // cross-wave merge over the two 64-row halves (wm = 0, 1)
+        __syncthreads();
+        uint64_t *s_top = (uint64_t *)smem;    // [2 wn][64 cols][K]
+        if (wm == 1 && half == 0) {
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+                for (int r = 0; r < K; ++r)
+                    s_top[(wn * 64 + bn * 32 + l32) * K + r] = top[bn][r];
+        }
+        __syncthreads();
+        if (wm == 0 && half == 0) {
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn) {
+                const int j = wn * 64 + bn * 32 + l32;
+                uint64_t other[K];
+#pragma unroll
+                for (int r = 0; r < K; ++r) other[r] = s_top[j * K + r];
+                mtip::topk_merge<K>(top[bn], other);
+                const int64_t gi = b0 + j;
+                if (gi < NB) {
+                    uint64_t *dst = part + (gi * n_tiles_a + ta) * K;
+#pragma unroll
+                    for (int r = 0; r < K; ++r) dst[r] = top[bn][r];
+                }
+            }
+        }
+    }
+}
+
+// One wave per row: K smallest keys of `n_keys` packed keys (KEYS = true)
+// or of `n_keys` floats (KEYS = false, key = (value, column)).
+template <int K, bool KEYS>
+__global__ __launch_bounds__(256) void k_topk_rows(const uint64_t *__restrict__ keys,
+                                                   const float *__restrict__ vals,
+                                                   int64_t n_rows, int64_t n_keys, int64_t ld,
+                                                   int K_out, int32_t *__restrict__ idx,
+                                                   float *__restrict__ val)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n_rows) return;
+    uint64_t t[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) t[r] = KEY_NONE;
+    for (int64_t e = lane; e < n_keys; e += 64) {
+        uint64_t k = KEYS ? keys[row * ld + e] : mtip::make_key(vals[row * ld + e], (uint32_t)e);
+        mtip::topk_insert<K>(t, k);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint64_t o[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r) o[r] = mtip::shfl_xor_u64(t[r], m);
+        mtip::topk_merge<K>(t, o);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            if (r >= K_out) break;
+            bool none = t[r] == KEY_NONE;
+            if (idx) idx[row * K_out + r] = none ? -1 : (int32_t)(uint32_t)t[r];
+            if (val) val[row * K_out + r] = none ? __builtin_nanf("") : mtip::ordered_to_float((uint32_t)(t[r] >> 32));
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f32(float v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Block per image: full distance to each of K candidates in FP32 and FP64.
+template <int K>
+__global__ __launch_bounds__(256) void k_rescore(const float *__restrict__ S, int64_t lds,
+                                                 int64_t N_o, const float *__restrict__ D, int64_t ldd,
+                                                 int64_t N_p, const double *__restrict__ alpha,
+                                                 const int32_t *__restrict__ cand,
+                                                 int32_t *__restrict__ best_idx,
+                                                 double *__restrict__ best_d64,
+                                                 float *__restrict__ d32, double *__restrict__ d64)
+{
+    __shared__ double s64[4][K];
+    __shared__ float s32[4][K];
+    const int64_t i = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double al = *alpha;
+    const float alf = (float)al;
+    int32_t c[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        int32_t v = cand[i * K + r];
+        c[r] = (v >= 0 && v < N_o) ? v : -1;
+    }
+    double a64[K];
+    float a32[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) { a64[r] = 0.0; a32[r] = 0.0f; }
+    const float *d = D + i * ldd;
+    const bool vec = (N_p % 4 == 0) && (ldd % 4 == 0) && (lds % 4 == 0);
+    if (vec) {
+        for (int64_t p = 4 * tid; p < N_p; p += 1024) {
+            floatx4 dv = *(const floatx4 *)(d + p);
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                if (c[r] < 0) continue;
+                floatx4 sv = *(const floatx4 *)(S + (int64_t)c[r] * lds + p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    double e = al * (double)sv[t] - (double)dv[t];
+                    a64[r] += e * e;
+                    float ef = fmaf(alf, sv[t], -dv[t]);
+                    a32[r] = fmaf(ef, ef, a32[r]);
+                }
+            }
+        }
+    } else {
+        for (int64_t p = tid; p < N_p; p += 256) {
+            float dv = d[p];
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                if (c[r] < 0) continue;
+                float sv = S[(int64_t)c[r] * lds + p];
+                double e = al * (double)sv - (double)dv;
+                a64[r] += e * e;
+                float ef = fmaf(alf, sv, -dv);
+                a32[r] = fmaf(ef, ef, a32[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        double v64 = wave_sum_f64(a64[r]);
+        float v32 = wave_sum_f32(a32[r]);
+        if (lane == 0) { s64[wave][r] = v64; s32[wave][r] = v32; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int32_t bi = -1;
+        double bd = 0.0;
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            double t64 = s64[0][r] + s64[1][r] + s64[2][r] + s64[3][r];
+            float t32 = (s32[0][r] + s32[1][r]) + (s32[2][r] + s32[3][r]);
+            if (c[r] < 0) { t64 = __builtin_nan(""); t32 = __builtin_nanf(""); }
+            if (d64) d64[i * K + r] = t64;
+            if (d32) d32[i * K + r] = t32;
+            if (c[r] >= 0 && (bi < 0 || t64 < bd || (t64 == bd && c[r] < bi))) {
+                bi = c[r];
+                bd = t64;
+            }
+        }
+        if (best_idx) best_idx[i] = bi;
+        if (best_d64) best_d64[i] = bi < 0 ? __builtin_nan("") : bd;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_row_stats(const float *__restrict__ in, int64_t cols,
+                                                   int64_t ld, double *__restrict__ rs,
+                                                   double *__restrict__ rq)
+{
+    __shared__ double sa[4], sb[4];
+    const int64_t r = blockIdx.x;
+    double s = 0.0, q = 0.0;
+    for (int64_t c = threadIdx.x; c < cols; c += 256) {
+        double v = in[r * ld + c];
+        s += v;
+        q += v * v;
+    }
+    s = wave_sum_f64(s);
+    q = wave_sum_f64(q);
+    if ((threadIdx.x & 63) == 0) { sa[threadIdx.x >> 6] = s; sb[threadIdx.x >> 6] = q; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (rs) rs[r] = (sa[0] + sa[1]) + (sa[2] + sa[3]);
+        if (rq) rq[r] = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scaling_ratio(const double *__restrict__ is,
+                                                       const double *__restrict__ iq, int64_t ni,
+                                                       const double *__restrict__ ss,
+                                                       const double *__restrict__ sq, int64_t ns,
+                                                       int64_t n_pix, double *__restrict__ alpha)
+{
+    __shared__ double red[4][4];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int64_t k = threadIdx.x; k < ni; k += 256) { a += is[k]; b += iq[k]; }
+    for (int64_t k = threadIdx.x; k < ns; k += 256) { c += ss[k]; d += sq[k]; }
+    a = wave_sum_f64(a);
+    b = wave_sum_f64(b);
+    c = wave_sum_f64(c);
+    d = wave_sum_f64(d);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w][0] = a; red[w][1] = b; red[w][2] = c; red[w][3] = d; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4];
+        for (int k = 0; k < 4; ++k) t[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        double Ni = (double)ni * (double)n_pix, Ns = (double)ns * (double)n_pix;
+        double mi = t[0] / Ni, ms = t[2] / Ns;
+        double vi = fmax(t[1] / Ni - mi * mi, 0.0), vs = fmax(t[3] / Ns - ms * ms, 0.0);
+        *alpha = sqrt(vi) / sqrt(vs);
+    }
+}
+
+__device__ __forceinline__ uint16_t bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ float block_max_f32(float m, float *red)
+{
+    for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// Block per row: r = m * x - c (m from a device scalar or its inverse, c a
+// per-column centre), cast to FMT, FP64 sum of squares of the values the
+// GEMM will see (dequantised), FP8 per-row scale absmax / 448.  Columns in
+// [cols, ld_out) are written as zero.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_residual(const float *__restrict__ in, int64_t cols,
+                                                  int64_t ld_in, const double *__restrict__ mult,
+                                                  int invert, const float *__restrict__ center,
+                                                  void *__restrict__ out, int64_t ld_out,
+                                                  float *__restrict__ scales,
+                                                  double *__restrict__ sumsq)
+{
+    __shared__ float redf[4];
+    __shared__ double redd[4];
+    const int64_t r = blockIdx.x;
+    const float *src = in + r * ld_in;
+    float m = 1.0f;
+    if (mult) m = invert ? (float)(1.0 / *mult) : (float)(*mult);
+    float scale = 1.0f, inv = 1.0f;
+    if (FMT == MTIP_FP8) {
+        float a = 0.0f;
+        for (int64_t c = threadIdx.x; c < cols; c += 256)
+            a = fmaxf(a, fabsf(center ? fmaf(m, src[c], -center[c]) : m * src[c]));
+        a = block_max_f32(a, redf);
+        scale = (a > 0.0f && isfinite(a)) ? a / 448.0f : 1.0f;
+        inv = 1.0f / scale;
+        if (threadIdx.x == 0 && scales) scales[r] = scale;
+    }
+    double ss = 0.0;
+    if (FMT == MTIP_FP8) {
+        uint8_t *dst = (uint8_t *)out + r * ld_out;
+        for (int64_t c4 = 4 * (int64_t)threadIdx.x; c4 < ld_out; c4 += 1024) {
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                int64_t c = c4 + t;
+                float x = 0.0f;
+                if (c < cols) x = (center ? fmaf(m, src[c], -center[c]) : m * src[c]) * inv;
+                v[t] = fminf(fmaxf(x, -448.0f), 448.0f);
+            }
+            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+            const float dqv[4] = {__builtin_amdgcn_cvt_f32_fp8(pk, 0), __builtin_amdgcn_cvt_f32_fp8(pk, 1),
+                                  __builtin_amdgcn_cvt_f32_fp8(pk, 2), __builtin_amdgcn_cvt_f32_fp8(pk, 3)};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (c4 + t < ld_out) {
+                    dst[c4 + t] = (uint8_t)((uint32_t)pk >> (8 * t));
+                    double dq = (double)dqv[t] * (double)scale;
+                    ss += dq * dq;
+                }
+            }
+        }
+    } else {
+        for (int64_t c = threadIdx.x; c < ld_out; c += 256) {
+            float x = 0.0f;
+            if (c < cols) x = center ? fmaf(m, src[c], -center[c]) : m * src[c];
+            if (FMT == MTIP_BF16) {
+                uint16_t b = bf16_rne(x);
+                ((uint16_t *)out)[r * ld_out + c] = b;
+                double dq = bf16_to_f32(b);
+                ss += dq * dq;
+            } else {
+                ((float *)out)[r * ld_out + c] = x;
+                ss += (double)x * (double)x;
+            }
+        }
+    }
+    if (sumsq) {
+        ss = wave_sum_f64(ss);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) redd[threadIdx.x >> 6] = ss;
+        __syncthreads();
+        if (threadIdx.x == 0) sumsq[r] = (redd[0] + redd[1]) + (redd[2] + redd[3]);
+    }
+}
+
+// Column means (FP64 accumulation) of a float32 (rows, cols) matrix: block of
+// 256 = 64 columns x 4 row phases, partial sums combined in LDS.
+__global__ __launch_bounds__(256) void k_col_mean(const float *__restrict__ in, int64_t rows,
+                                                  int64_t cols, int64_t ld, float *__restrict__ mean)
+{
+    __shared__ double part[4][64];
+    const int cx = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * 64 + cx;
+    double s = 0.0;
+    if (c < cols)
+        for (int64_t r = ph; r < rows; r += 4) s += in[r * ld + c];
+    part[ph][cx] = s;
+    __syncthreads();
+    if (ph == 0 && c < cols)
+        mean[c] = (float)(((part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx])) / (double)rows);
+}
+
+template <int PREC, int K>
+void launch_gemm_topk(const void *S, int64_t lds_b, int64_t N_o, const void *D, int64_t ldd_b,
+                      int64_t N_i, int64_t kbytes, const float *ssc, const float *dsc,
+                      const double *s2, const double *alpha, int nta, int ntb, uint64_t *part,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL((k_gemm<PREC, K, OUT_TOPK>), dim3((unsigned)(nta * ntb)), dim3(256), 0, st,
+                       (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i, kbytes, ssc,
+                       dsc, s2, alpha, nta, part, nullptr, (int64_t)0);
+}
+
+template <int K>
+void launch_topk_rows(const uint64_t *keys, const float *vals, int64_t rows, int64_t n, int64_t ld,
+                      int kout, int32_t *idx, float *val, hipStream_t st)
+{
+    dim3 g((unsigned)mtip::ceil_div(rows, 4));
+    if (keys)
+        hipLaunchKernelGGL((k_topk_rows<K, true>), g, dim3(256), 0, st, keys, vals, rows, n, ld,
+                           kout, idx, val);
+    else
+        hipLaunchKernelGGL((k_topk_rows<K, false>), g, dim3(256), 0, st, keys, vals, rows, n, ld,
+                           kout, idx, val);
+}
+
+int prec_bytes(int p) { return p == MTIP_FP32 ? 4 : p == MTIP_BF16 ? 2 : p == MTIP_FP8 ? 1 : 0; }
+
+}  // namespace
+
+extern "C" size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision)
+{
+    (void)precision;
+    if (N_o <= 0 || N_i <= 0 || K <= 0) return 0;
+    int64_t nta = mtip::ceil_div(N_o, BM);
+    return (size_t)(N_i * nta * K) * sizeof(uint64_t);
+}
+
+extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
+                               const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                               const void *images, int64_t ld_d, const float *image_scale,
+                               int64_t N_i, int64_t N_p, const double *alpha, int K,
+                               void *workspace, size_t workspace_bytes, int32_t *top_idx,
+                               float *top_val, void *stream)
+{
+    const int eb = prec_bytes(precision);
+    MTIP_REQUIRE(eb > 0, MTIP_ERR_ARG, "mtip_match_topk: unknown precision");
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_match_topk: K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(N_o >= 0 && N_i >= 0 && N_p >= 0, MTIP_ERR_ARG, "mtip_match_topk: negative size");
+    MTIP_REQUIRE(N_o <= 0x7fffffff, MTIP_ERR_ARG, "mtip_match_topk: N_o too large");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(N_o > 0, MTIP_ERR_ARG, "mtip_match_topk: no orientations");
+    MTIP_REQUIRE(N_p > 0 && N_p % 16 == 0, MTIP_ERR_ARG, "mtip_match_topk: N_p must be a positive multiple of 16");
+    MTIP_REQUIRE(ld_s >= N_p && ld_d >= N_p && (ld_s * eb) % 16 == 0 && (ld_d * eb) % 16 == 0,
+                 MTIP_ERR_ARG, "mtip_match_topk: leading dimensions must be >= N_p and 16-byte multiples");
+    MTIP_REQUIRE(slices && images && slice_sumsq && alpha && top_idx, MTIP_ERR_ARG,
+                 "mtip_match_topk: null pointer");
+    MTIP_REQUIRE(precision != MTIP_FP8 || (slice_scale && image_scale), MTIP_ERR_ARG,
+                 "mtip_match_topk: FP8 needs per-row scales");
+    MTIP_REQUIRE(((uintptr_t)slices % 16) == 0 && ((uintptr_t)images % 16) == 0, MTIP_ERR_ARG,
+                 "mtip_match_topk: operands must be 16-byte aligned");
+    size_t need = mtip_match_workspace_size(N_o, N_i, K, precision);
+    MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM,
+                 "mtip_match_topk: workspace too small");
+    const int nta = (int)mtip::ceil_div(N_o, BM);
+    const int64_t ntb = mtip::ceil_div(N_i, BN);
+    MTIP_REQUIRE((int64_t)nta * ntb < 0x7fffffff, MTIP_ERR_ARG, "mtip_match_topk: grid too large");
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t *part = (uint64_t *)workspace;
+    const int64_t kbytes = N_p * eb;
+#define MTIP_GEMM_CASE(P)                                                                        \
+    switch (K) {                                                                                 \
+    case 1: launch_gemm_topk<P, 1>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   part, st); break;                                             \
+    case 2: launch_gemm_topk<P, 2>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   part, st); break;                                             \
+    case 4: launch_gemm_topk<P, 4>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   part, st); break;                                             \
+    default: launch_gemm_topk<P, 8>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,    \
+                                    slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb, \
+                                    part, st); break;                                            \
+    }
+    if (precision == MTIP_FP32) { MTIP_GEMM_CASE(MTIP_FP32) }
+    else if (precision == MTIP_BF16) { MTIP_GEMM_CASE(MTIP_BF16) }
+    else { MTIP_GEMM_CASE(MTIP_FP8) }
+#undef MTIP_GEMM_CASE
+    MTIP_HIP_CHECK(hipGetLastError());
+    const int64_t nkeys = (int64_t)nta * K;
+    switch (K) {
+    case 1: launch_topk_rows<1>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
+    case 2: launch_topk_rows<2>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
+    case 4: launch_topk_rows<4>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
+    default: launch_topk_rows<8>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
+    }
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
+                            int64_t N_i, int64_t N_p, const double *alpha, const int32_t *cand,
+                            int K, int32_t *best_idx, double *best_d64, float *d32, double *d64,
+                            void *stream)
+{
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_rescore: K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(N_i >= 0 && N_p >= 0, MTIP_ERR_ARG, "mtip_rescore: negative size");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(N_i <= 0x7fffffff, MTIP_ERR_ARG, "mtip_rescore: N_i too large");
+    MTIP_REQUIRE(slices && images && alpha && cand, MTIP_ERR_ARG, "mtip_rescore: null pointer");
+    MTIP_REQUIRE(ld_s >= N_p && ld_d >= N_p, MTIP_ERR_ARG, "mtip_rescore: leading dimension < N_p");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 g((unsigned)N_i);
+    switch (K) {
+    case 1: hipLaunchKernelGGL(k_rescore<1>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
+    case 2: hipLaunchKernelGGL(k_rescore<2>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
+    case 4: hipLaunchKernelGGL(k_rescore<4>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
+    default: hipLaunchKernelGGL(k_rescore<8>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
+    }
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_row_stats(const float *in, int64_t rows, int64_t cols, int64_t ld,
+                              double *row_sum, double *row_sumsq, void *stream)
+{
+    MTIP_REQUIRE(rows >= 0 && cols >= 0 && ld >= cols, MTIP_ERR_ARG, "mtip_row_stats: bad shape");
+    if (rows == 0) return MTIP_OK;
+    MTIP_REQUIRE(rows <= 0x7fffffff, MTIP_ERR_ARG, "mtip_row_stats: too many rows");
+    MTIP_REQUIRE(in && (row_sum || row_sumsq), MTIP_ERR_ARG, "mtip_row_stats: null pointer");
+    hipLaunchKernelGGL(k_row_stats, dim3((unsigned)rows), dim3(256), 0, (hipStream_t)stream, in,
+                       cols, ld, row_sum, row_sumsq);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_scaling_ratio(const double *img_sum, const double *img_sumsq, int64_t n_img,
+                                  const double *sl_sum, const double *sl_sumsq, int64_t n_sl,
+                                  int64_t n_pix, double *alpha, void *stream)
+{
+    MTIP_REQUIRE(n_img > 0 && n_sl > 0 && n_pix > 0, MTIP_ERR_ARG, "mtip_scaling_ratio: empty input");
+    MTIP_REQUIRE(img_sum && img_sumsq && sl_sum && sl_sumsq && alpha, MTIP_ERR_ARG,
+                 "mtip_scaling_ratio: null pointer");
+    hipLaunchKernelGGL(k_scaling_ratio, dim3(1), dim3(256), 0, (hipStream_t)stream, img_sum,
+                       img_sumsq, n_img, sl_sum, sl_sumsq, n_sl, n_pix, alpha);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in,
+                                  const double *mult, int invert_mult, const float *center,
+                                  int format, void *out, int64_t ld_out, float *scales,
+                                  double *sumsq, void *stream)
+{
+    MTIP_REQUIRE(rows >= 0 && cols >= 0 && ld_in >= cols && ld_out >= cols, MTIP_ERR_ARG,
+                 "mtip_residual_rows: bad shape");
+    if (rows == 0) return MTIP_OK;
+    MTIP_REQUIRE(rows <= 0x7fffffff, MTIP_ERR_ARG, "mtip_residual_rows: too many rows");
+    MTIP_REQUIRE(in && out, MTIP_ERR_ARG, "mtip_residual_rows: null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 g((unsigned)rows);
+    if (format == MTIP_FP32)
+        hipLaunchKernelGGL(k_residual<MTIP_FP32>, g, dim3(256), 0, st, in, cols, ld_in, mult,
+                           invert_mult, center, out, ld_out, scales, sumsq);
+    else if (format == MTIP_BF16)
+        hipLaunchKernelGGL(k_residual<MTIP_BF16>, g, dim3(256), 0, st, in, cols, ld_in, mult,
+                           invert_mult, center, out, ld_out, scales, sumsq);
+    else if (format == MTIP_FP8) {
+        MTIP_REQUIRE(scales, MTIP_ERR_ARG, "mtip_residual_rows: FP8 needs a scales array");
+        hipLaunchKernelGGL(k_residual<MTIP_FP8>, g, dim3(256), 0, st, in, cols, ld_in, mult,
+                           invert_mult, center, out, ld_out, scales, sumsq);
+    } else {
+        return mtip::fail(MTIP_ERR_ARG, "mtip_residual_rows: unknown format");
+    }
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_quantize_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in,
+                                  int format, void *out, int64_t ld_out, float *scales,
+                                  void *stream)
+{
+    MTIP_REQUIRE(format == MTIP_BF16 || format == MTIP_FP8, MTIP_ERR_ARG,
+                 "mtip_quantize_rows: format must be MTIP_BF16 or MTIP_FP8");
+    return mtip_residual_rows(in, rows, cols, ld_in, nullptr, 0, nullptr, format, out, ld_out,
+                              scales, nullptr, stream);
+}
+
+extern "C" int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, float *mean,
+                             void *stream)
+{
+    MTIP_REQUIRE(rows > 0 && cols >= 0 && ld >= cols, MTIP_ERR_ARG, "mtip_col_mean: bad shape");
+    if (cols == 0) return MTIP_OK;
+    MTIP_REQUIRE(in && mean, MTIP_ERR_ARG, "mtip_col_mean: null pointer");
+    hipLaunchKernelGGL(k_col_mean, dim3((unsigned)mtip::ceil_div(cols, 64)), dim3(256), 0,
+                       (hipStream_t)stream, in, rows, cols, ld, mean);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_euclidean_distance(const float *ref, const float *data, int64_t numRef,
+                                       int64_t numData, int64_t totalPixels, float *euDist,
+                                       void *stream)
+{
+    MTIP_REQUIRE(numRef >= 0 && numData >= 0 && totalPixels >= 0, MTIP_ERR_ARG,
+                 "mtip_euclidean_distance: negative size");
+    if (numRef == 0 || numData == 0) return MTIP_OK;
+    MTIP_REQUIRE(totalPixels > 0 && totalPixels % 32 == 0, MTIP_ERR_ARG,
+                 "mtip_euclidean_distance: totalPixels must be a positive multiple of 32");
+    MTIP_REQUIRE(ref && data && euDist, MTIP_ERR_ARG, "mtip_euclidean_distance: null pointer");
+    MTIP_REQUIRE(((uintptr_t)ref % 16) == 0 && ((uintptr_t)data % 16) == 0, MTIP_ERR_ARG,
+                 "mtip_euclidean_distance: inputs must be 16-byte aligned");
+    const int nta = (int)mtip::ceil_div(numRef, BM);
+    const int64_t ntb = mtip::ceil_div(numData, BN);
+    MTIP_REQUIRE((int64_t)nta * ntb < 0x7fffffff, MTIP_ERR_ARG, "mtip_euclidean_distance: grid too large");
+    hipLaunchKernelGGL((k_gemm<MTIP_FP32, 1, OUT_FULL>), dim3((unsigned)(nta * ntb)), dim3(256), 0,
+                       (hipStream_t)stream, (const uint8_t *)ref, totalPixels * 4, numRef,
+                       (const uint8_t *)data, totalPixels * 4, numData, totalPixels * 4, nullptr,
+                       nullptr, nullptr, nullptr, nta, nullptr, euDist, numRef);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_heap_sort(const float *euDist, int64_t numData, int64_t numRef, int totalNN,
+                              int32_t *idx, void *stream)
+{
+    MTIP_REQUIRE(numData >= 0 && numRef >= 0, MTIP_ERR_ARG, "mtip_heap_sort: negative size");
+    MTIP_REQUIRE(totalNN >= 1 && totalNN <= 8, MTIP_ERR_ARG, "mtip_heap_sort: totalNN must be 1..8");
+    if (numData == 0) return MTIP_OK;
+    MTIP_REQUIRE(numRef <= 0x7fffffff, MTIP_ERR_ARG, "mtip_heap_sort: numRef too large");
+    MTIP_REQUIRE(euDist && idx, MTIP_ERR_ARG, "mtip_heap_sort: null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if (totalNN == 1)
+        launch_topk_rows<1>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
+    else if (totalNN == 2)
+        launch_topk_rows<2>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
+    else if (totalNN <= 4)
+        launch_topk_rows<4>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
+    else
+        launch_topk_rows<8>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}

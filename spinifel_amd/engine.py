@@ -1,0 +1,402 @@
+"""Device-resident orientation-matching pipeline over libmtip.
+
+PyTorch provides device memory and the current HIP stream; every compute
+step is a libmtip entry point.  Nothing here falls back to the CPU: a
+missing library or GPU raises.
+
+Pipeline of one ``match`` (all on one stream, inputs resident in HBM):
+
+1. ``NufftPlan.set_model``  -- symmetrise / deconvolve / fine grid (once per model)
+2. ``NufftPlan.slices``     -- FP64 rotation + ES interpolation -> FP32 slices
+                               (+ optional bf16 copy) and per-orientation sums
+3. ``ImageSet``             -- images resident as FP32 (+ bf16 / e4m3 copies)
+                               with per-row sums
+4. ``scaling_ratio``        -- alpha = std(images) / std(slices) on the device
+5. ``topk``                 -- MFMA cross term + fused top-K per image
+6. ``rescore``              -- FP32 and FP64 full distances of the candidates
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_PAD = 16      # pixel padding multiple for the GEMM operands
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("spinifel_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    _lib.load()
+
+
+def padded(n, mult=_PAD):
+    return ((int(n) + mult - 1) // mult) * mult
+
+
+class NufftPlan:
+    """Type-2 NUFFT plan of a cubic M^3 model (ES kernel, sigma = 2)."""
+
+    def __init__(self, M, eps=1e-6, device=0):
+        require_gpu()
+        self.lib = _lib.load()
+        self.M = int(M)
+        self.eps = float(eps)
+        self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.mtip_plan_create(self.M, self.eps, self.device.index or 0,
+                                             ctypes.byref(h)), "mtip_plan_create")
+        self._h = h
+        n, w, np_ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        beta = ctypes.c_double()
+        _lib.check(self.lib.mtip_plan_info(h, ctypes.byref(n), ctypes.byref(w),
+                                           ctypes.byref(beta), ctypes.byref(np_)), "mtip_plan_info")
+        self.n_fine, self.width, self.beta, self.padded_edge = n.value, w.value, beta.value, np_.value
+        self._model = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.mtip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, ac, stream=None):
+        """ac: (M, M, M) real array or tensor; copied to the device as float32."""
+        if isinstance(ac, np.ndarray):
+            if np.iscomplexobj(ac):
+                if not np.all(np.isreal(ac)):
+                    raise AssertionError("model must be real (use_recip_sym)")
+                ac = ac.real
+            t = torch.from_numpy(np.ascontiguousarray(ac, dtype=np.float32)).to(self.device)
+        else:
+            t = ac.to(device=self.device, dtype=torch.float32).contiguous()
+        if tuple(t.shape) != (self.M,) * 3:
+            raise ValueError(f"model shape {tuple(t.shape)} != {(self.M,) * 3}")
+        self._model = t
+        _lib.check(self.lib.mtip_plan_set_model(self._h, _ptr(t), _stream(stream)),
+                   "mtip_plan_set_model")
+
+    def execute(self, x, y, z, stream=None, check_domain=True):
+        """Re of the type-2 transform at points (x, y, z) (radians) -> float32 tensor."""
+        xs = [torch.as_tensor(v, dtype=torch.float64).reshape(-1).to(self.device).contiguous()
+              for v in (x, y, z)]
+        N = xs[0].numel()
+        if not (xs[1].numel() == N and xs[2].numel() == N):
+            raise ValueError("x, y, z differ in length")
+        out = torch.empty(N, dtype=torch.float32, device=self.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.mtip_plan_execute(self._h, _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]), N,
+                                              _ptr(out), _ptr(bad), _stream(stream)),
+                   "mtip_plan_execute")
+        if check_domain and int(bad.item()) > 0:
+            raise AssertionError(f"{int(bad.item())} NU points outside [-3pi, 3pi)")
+        return out
+
+    def slices(self, quats, q, scale, out=None, ld=None, want_bf16=False, bad=None, stream=None):
+        """Slices at N_o orientations for pixel q-vectors q (3, N_p).
+
+        quats: (N_o, 4) float64 device tensor; q: (3, N_p) float64 device tensor.
+        Returns dict(s32 (N_o, ld) float32, bf16 (N_o, ld) int16 bits or None,
+        row_sum, row_sumsq (N_o,) float64, bad (1,) int32).  Pixel columns
+        beyond N_p are zero.
+        """
+        quats = quats.to(self.device, torch.float64).contiguous()
+        q = q.to(self.device, torch.float64).contiguous()
+        N_o = quats.shape[0]
+        N_p = q.shape[1]
+        if ld is None:
+            ld = padded(N_p)
+        if out is None:
+            out = torch.zeros((N_o, ld), dtype=torch.float32, device=self.device)
+        bf = torch.zeros((N_o, ld), dtype=torch.int16, device=self.device) if want_bf16 else None
+        rs = torch.empty(N_o, dtype=torch.float64, device=self.device)
+        rq = torch.empty(N_o, dtype=torch.float64, device=self.device)
+        if bad is None:
+            bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        _lib.check(self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, float(scale),
+                                             _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
+                                             _stream(stream)), "mtip_plan_slices")
+        return dict(s32=out, bf16=bf, row_sum=rs, row_sumsq=rq, bad=bad, N_p=N_p, ld=ld)
+
+
+def row_stats(x, n_cols=None, stream=None):
+    """FP64 per-row sum and sum of squares of a float32 (rows, ld) tensor."""
+    lib = _lib.load()
+    rows, ld = x.shape
+    cols = ld if n_cols is None else n_cols
+    rs = torch.empty(rows, dtype=torch.float64, device=x.device)
+    rq = torch.empty(rows, dtype=torch.float64, device=x.device)
+    _lib.check(lib.mtip_row_stats(_ptr(x), rows, cols, ld, _ptr(rs), _ptr(rq), _stream(stream)),
+               "mtip_row_stats")
+    return rs, rq
+
+
+def quantize(x, fmt, n_cols=None, stream=None):
+    """Cast (rows, ld) float32 rows to bf16 bits (int16) or e4m3 (uint8 + scales)."""
+    lib = _lib.load()
+    rows, ld = x.shape
+    cols = ld if n_cols is None else n_cols
+    if fmt == "bf16":
+        out = torch.empty((rows, ld), dtype=torch.int16, device=x.device)
+        sc = None
+        code = _lib.MTIP_BF16
+    elif fmt == "fp8":
+        out = torch.empty((rows, ld), dtype=torch.uint8, device=x.device)
+        sc = torch.empty(rows, dtype=torch.float32, device=x.device)
+        code = _lib.MTIP_FP8
+    else:
+        raise ValueError(fmt)
+    _lib.check(lib.mtip_quantize_rows(_ptr(x), rows, cols, ld, code, _ptr(out), ld, _ptr(sc),
+                                      _stream(stream)), "mtip_quantize_rows")
+    return out, sc
+
+
+def scaling_ratio(img_sum, img_sumsq, sl_sum, sl_sumsq, n_pix, stream=None):
+    lib = _lib.load()
+    alpha = torch.empty(1, dtype=torch.float64, device=img_sum.device)
+    _lib.check(lib.mtip_scaling_ratio(_ptr(img_sum), _ptr(img_sumsq), img_sum.numel(),
+                                      _ptr(sl_sum), _ptr(sl_sumsq), sl_sum.numel(), int(n_pix),
+                                      _ptr(alpha), _stream(stream)), "mtip_scaling_ratio")
+    return alpha
+
+
+@dataclass
+class Operand:
+    """A row matrix resident in one GEMM operand format: data (rows, ld),
+    per-row dequantisation scales (e4m3 only) and FP64 squared row norms of
+    the values as cast."""
+    data: torch.Tensor
+    scale: torch.Tensor = None
+    sumsq: torch.Tensor = None
+
+
+_FMT = {"fp32": (_lib.MTIP_FP32, torch.float32), "bf16": (_lib.MTIP_BF16, torch.int16),
+        "fp8": (_lib.MTIP_FP8, torch.uint8)}
+
+
+def residual(x, precision, n_cols=None, mult=None, invert=False, center=None, stream=None):
+    """Operand of rows ``cast(m * x - center)`` (m = mult or 1 / mult, a device scalar)."""
+    lib = _lib.load()
+    rows, ld = x.shape
+    cols = ld if n_cols is None else n_cols
+    code, dt = _FMT[precision]
+    out = torch.empty((rows, ld), dtype=dt, device=x.device)
+    sc = torch.empty(rows, dtype=torch.float32, device=x.device) if precision == "fp8" else None
+    ss = torch.empty(rows, dtype=torch.float64, device=x.device)
+    _lib.check(lib.mtip_residual_rows(_ptr(x), rows, cols, ld, _ptr(mult), int(bool(invert)),
+                                      _ptr(center), code, _ptr(out), ld, _ptr(sc), _ptr(ss),
+                                      _stream(stream)), "mtip_residual_rows")
+    return Operand(out, sc, ss)
+
+
+def col_mean(x, n_cols=None, stream=None):
+    """Column means of (rows, ld) float32 rows -> (ld,) float32 (zeros past n_cols)."""
+    lib = _lib.load()
+    rows, ld = x.shape
+    cols = ld if n_cols is None else n_cols
+    out = torch.zeros(ld, dtype=torch.float32, device=x.device)
+    _lib.check(lib.mtip_col_mean(_ptr(x), rows, cols, ld, _ptr(out), _stream(stream)),
+               "mtip_col_mean")
+    return out
+
+
+class ImageSet:
+    """Images resident in HBM: FP32 rows padded to a multiple of 16 pixels,
+    FP64 per-row sums, and lazily built bf16 / e4m3 copies."""
+
+    def __init__(self, images, device=0, stream=None):
+        require_gpu()
+        dev = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+        if isinstance(images, np.ndarray):
+            arr = np.ascontiguousarray(images, dtype=np.float32).reshape(images.shape[0], -1)
+            t = torch.from_numpy(arr).to(dev)
+        else:
+            t = images.to(dev, torch.float32).reshape(images.shape[0], -1)
+        self.N_i, self.N_p = t.shape
+        self.ld = padded(self.N_p)
+        if self.ld != self.N_p:
+            buf = torch.zeros((self.N_i, self.ld), dtype=torch.float32, device=dev)
+            buf[:, :self.N_p] = t
+            t = buf
+        self.f32 = t.contiguous()
+        self.device = dev
+        self.row_sum, self.row_sumsq = row_stats(self.f32, self.N_p, stream)
+        self._ops = {}
+
+    def operand(self, precision, alpha, center=None, stream=None):
+        """Images divided by alpha minus the slice-side centre, cast to `precision`."""
+        return residual(self.f32, precision, self.N_p, mult=alpha, invert=True, center=center,
+                        stream=stream)
+
+
+def workspace_bytes(N_o, N_i, K, precision):
+    return int(_lib.load().mtip_match_workspace_size(N_o, N_i, K, _lib.PRECISIONS[precision]))
+
+
+def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stream=None):
+    """Top-K orientations per image by v = alpha^2 ||s||^2 - 2 alpha s.d.
+
+    S, D: Operand; returns (idx int32 (N_i, K), val float32 (N_i, K)).
+    """
+    lib = _lib.load()
+    dev = D.data.device
+    need = workspace_bytes(N_o, N_i, K, precision)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+    idx = torch.empty((N_i, K), dtype=torch.int32, device=dev)
+    val = torch.empty((N_i, K), dtype=torch.float32, device=dev)
+    _lib.check(lib.mtip_match_topk(_lib.PRECISIONS[precision], _ptr(S.data), S.data.shape[1],
+                                   _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
+                                   D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
+                                   _ptr(workspace), workspace.numel(), _ptr(idx), _ptr(val),
+                                   _stream(stream)), "mtip_match_topk")
+    return idx, val
+
+
+def rescore(S32, D32, N_p, alpha, cand, stream=None):
+    """FP32 and FP64 distances of the candidates; FP64 argmin per image."""
+    lib = _lib.load()
+    N_i, K = cand.shape
+    dev = cand.device
+    best = torch.empty(N_i, dtype=torch.int32, device=dev)
+    bd = torch.empty(N_i, dtype=torch.float64, device=dev)
+    d32 = torch.empty((N_i, K), dtype=torch.float32, device=dev)
+    d64 = torch.empty((N_i, K), dtype=torch.float64, device=dev)
+    _lib.check(lib.mtip_rescore(_ptr(S32), S32.shape[1], S32.shape[0], _ptr(D32), D32.shape[1], N_i, N_p,
+                                _ptr(alpha), _ptr(cand.contiguous()), K, _ptr(best), _ptr(bd),
+                                _ptr(d32), _ptr(d64), _stream(stream)), "mtip_rescore")
+    return dict(best=best, best_d64=bd, d32=d32, d64=d64)
+
+
+def euclidean_distance(ref, data, stream=None):
+    """Squared distances (numData, numRef) float32 (cudaEuclideanDistance)."""
+    lib = _lib.load()
+    ref = ref.contiguous()
+    data = data.contiguous()
+    numRef, P = ref.shape
+    numData = data.shape[0]
+    out = torch.empty((numData, numRef), dtype=torch.float32, device=data.device)
+    _lib.check(lib.mtip_euclidean_distance(_ptr(ref), _ptr(data), numRef, numData, P, _ptr(out),
+                                           _stream(stream)), "mtip_euclidean_distance")
+    return out
+
+
+def heap_sort(euDist, totalNN, stream=None):
+    """Indices of the totalNN smallest per row (cudaHeapSort)."""
+    lib = _lib.load()
+    euDist = euDist.contiguous()
+    nd, nr = euDist.shape
+    idx = torch.empty((nd, totalNN), dtype=torch.int32, device=euDist.device)
+    _lib.check(lib.mtip_heap_sort(_ptr(euDist), nd, nr, int(totalNN), _ptr(idx), _stream(stream)),
+               "mtip_heap_sort")
+    return idx
+
+
+class SliceSet:
+    """Slices of one model at one orientation set, resident in HBM as FP32
+    rows (N_o, ld) with FP64 per-row sums, plus lazily built bf16 / e4m3
+    operand copies."""
+
+    def __init__(self, f32, N_p, row_sum, row_sumsq, bf16=None, quats=None, bad=None):
+        self.f32 = f32
+        self.N_o = f32.shape[0]
+        self.N_p = int(N_p)
+        self.row_sum, self.row_sumsq = row_sum, row_sumsq
+        self.quats = quats
+        self.bad = bad
+        self.bf16_plain = bf16
+        self._ops = {}
+        self._center = None
+
+    @classmethod
+    def compute(cls, plan, quats, q, scale, precision="fp32", stream=None):
+        """NUFFT slices of the plan's model at quaternions `quats` (N_o, 4)."""
+        dev = plan.device
+        quats = torch.as_tensor(np.ascontiguousarray(quats, np.float64)).to(dev) \
+            if isinstance(quats, np.ndarray) else quats.to(dev, torch.float64)
+        q = torch.as_tensor(np.ascontiguousarray(q, np.float64)).to(dev) \
+            if isinstance(q, np.ndarray) else q.to(dev, torch.float64)
+        r = plan.slices(quats, q, scale, stream=stream)
+        return cls(r["s32"], r["N_p"], r["row_sum"], r["row_sumsq"], r["bf16"], quats, r["bad"])
+
+    @classmethod
+    def from_rows(cls, rows, N_p=None, stream=None):
+        """Wrap given FP32 rows (N_o, ld) device tensor (ld a multiple of 16)."""
+        N_p = rows.shape[1] if N_p is None else N_p
+        rs, rq = row_stats(rows, N_p, stream)
+        return cls(rows, N_p, rs, rq)
+
+    @property
+    def center(self):
+        """Per-pixel mean slice (float32, ld), the centre both operands subtract."""
+        if self._center is None:
+            self._center = col_mean(self.f32, self.N_p)
+        return self._center
+
+    def operand(self, precision, center=True, stream=None):
+        """Slices minus the centre (or plain), cast to `precision`, cached."""
+        key = (precision, bool(center))
+        if key not in self._ops:
+            c = self.center if center else None
+            self._ops[key] = residual(self.f32, precision, self.N_p, center=c, stream=stream)
+        return self._ops[key]
+
+
+_ONES = {}
+
+
+def _one(dev):
+    if dev not in _ONES:
+        _ONES[dev] = torch.ones(1, dtype=torch.float64, device=dev)
+    return _ONES[dev]
+
+
+def match_resident(slices, images, precision="fp32", K=None, rescore_fp64=None, alpha=None,
+                   center=True, workspace=None, stream=None):
+    """Match resident images against resident slices.
+
+    Ranking uses the residuals s' = s - c and d' = d / alpha - c with c the
+    mean slice (``center=True``): ||s' - d'|| = ||alpha s - d|| / alpha, so
+    the order is the order of the true distances, while the cast to bf16 /
+    e4m3 only sees the orientation-dependent part.  precision 'fp32'
+    defaults to K = 1 without rescoring (the plain form); 'bf16' / 'fp8'
+    default to K = 8 with FP32 + FP64 rescoring of the raw FP32 data.
+    Returns a dict of device tensors: index (N_i,) int32 and diagnostics.
+    """
+    if K is None:
+        K = 1 if precision == "fp32" else 8
+    if rescore_fp64 is None:
+        rescore_fp64 = precision != "fp32"
+    if images.N_p != slices.N_p:
+        raise ValueError(f"images have {images.N_p} pixels, slices {slices.N_p}")
+    if alpha is None:
+        alpha = scaling_ratio(images.row_sum, images.row_sumsq, slices.row_sum, slices.row_sumsq,
+                              images.N_p, stream)
+    S = slices.operand(precision, center, stream)
+    D = images.operand(precision, alpha, slices.center if center else None, stream)
+    one = _one(alpha.device)
+    idx, val = topk(precision, S, S.sumsq, slices.N_o, D, images.N_i, images.N_p, one,
+                    K, workspace, stream)
+    out = dict(topk_idx=idx, topk_val=val, alpha=alpha)
+    if rescore_fp64:
+        r = rescore(slices.f32, images.f32, images.N_p, alpha, idx, stream)
+        out.update(index=r["best"], best_d64=r["best_d64"], d32=r["d32"], d64=r["d64"])
+    else:
+        out["index"] = idx[:, 0]
+    return out

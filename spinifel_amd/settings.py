@@ -1,0 +1,42 @@
+"""Settings shim: the fields of spinifel's settings singleton that the
+orientation-matching path reads.
+
+Reference: ``SpinifelSettings`` (spinifel/settings.py:167-848); the path
+reads ``N_orientations``, ``N_batch_size``, ``reduced_det_shape``
+(spinifel/sequential/orientation_matching.py:204-211) and ``oversampling``
+(:234-236).  The rest of the reference's TOML/CLI system is out of scope.
+
+Fields added for this implementation: ``nufft_eps`` (the reference hard-codes
+1e-12 for finufft, spinifel/autocorrelation.py:59; FP32 slices cannot carry
+more than ~1e-7 so the default here is 1e-6), ``precision`` of the cross-term
+GEMM, ``topk`` candidates and ``rescore`` for the precision forms, and
+``device``.
+"""
+from dataclasses import dataclass
+
+
+@dataclass
+class SpinifelSettings:
+    N_orientations: int = 10000
+    N_batch_size: int = 1000
+    reduced_det_shape: tuple = (1, 128, 128)
+    oversampling: int = 1
+    nufft_eps: float = 1e-6
+    precision: str = "fp32"     # 'fp32' | 'bf16' | 'fp8'
+    topk: int = 0               # 0: 1 for fp32, 8 otherwise
+    rescore: bool = None        # None: False for fp32, True otherwise
+    device: int = 0
+    verbose: bool = False
+
+    @property
+    def Mquat(self):
+        """int(oversampling * 20) (spinifel/settings.py:805-810)."""
+        return int(self.oversampling * 20)
+
+    @property
+    def M(self):
+        """4 * Mquat + 1 (spinifel/settings.py:813-818)."""
+        return 4 * self.Mquat + 1
+
+
+settings = SpinifelSettings()

@@ -1,0 +1,172 @@
+"""GPU parity of distances / top-K / rescoring / drop-in match against the
+oracle (FP64 numpy + scikit-learn formula)."""
+import numpy as np
+import pytest
+
+from helpers import workload
+from oracle import matching
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    from spinifel_amd import engine
+    engine.require_gpu()
+    return torch.device("cuda", 0)
+
+
+def _sliceset(rows, dev):
+    import torch
+    from spinifel_amd import engine
+    P = engine.padded(rows.shape[1])
+    t = torch.zeros((rows.shape[0], P), dtype=torch.float32, device=dev)
+    t[:, :rows.shape[1]] = torch.from_numpy(np.ascontiguousarray(rows, np.float32)).to(dev)
+    return engine.SliceSet.from_rows(t, rows.shape[1])
+
+
+def test_knn_contract_random(dev):
+    """tests/test_OM.py:51-76 of the reference, seeded: the GPU argmin equals
+    the argmin of scikit-learn's float64 squared distances."""
+    import torch
+    from sklearn.metrics.pairwise import euclidean_distances
+    from spinifel_amd import engine
+    print("case: test_OM contract 512 x 1024 x 16384", flush=True)
+    rng = np.random.default_rng(0)
+    model = rng.random((512, 128 * 128))
+    data = rng.random((1024, 128 * 128))
+    cpu = np.argmin(euclidean_distances(model, data, squared=True), axis=0)
+    # KNN pair: full matrix + heap sort
+    ref_t = torch.from_numpy(model.astype(np.float32)).to(dev)
+    dat_t = torch.from_numpy(data.astype(np.float32)).to(dev)
+    eu = engine.euclidean_distance(ref_t, dat_t)
+    ref_full = euclidean_distances(model.astype(np.float32).astype(np.float64),
+                                   data.astype(np.float32).astype(np.float64), squared=True).T
+    np.testing.assert_allclose(eu.cpu().numpy(), ref_full, rtol=2e-4, atol=1e-2)
+    idx = engine.heap_sort(eu, 1).cpu().numpy()[:, 0]
+    # fused MFMA top-K + FP64 rescore (the path match uses)
+    S = _sliceset(model, dev)
+    D = engine.ImageSet(data, dev)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    r = engine.match_resident(S, D, "fp32", K=8, rescore_fp64=True, alpha=one)
+    fused = r["index"].cpu().numpy()
+    assert (fused == cpu).all()
+    agree_heap = (idx == cpu).mean()
+    print("heap-sort agreement", agree_heap)
+    assert agree_heap >= 0.999
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
+@pytest.mark.parametrize("shape", [(300, 77, 1024), (1000, 515, 2048), (128, 128, 256)])
+def test_topk_sets_and_rescore(dev, precision, shape):
+    import torch
+    from spinifel_amd import engine
+    N_o, N_i, N_p = shape
+    print(f"case: topk {precision} {shape}", flush=True)
+    rng = np.random.default_rng(N_o + N_i)
+    S = rng.gamma(2.0, 1.0, (N_o, N_p))
+    idx_true = rng.integers(0, N_o, N_i)
+    Dm = rng.poisson(S[idx_true] * 1.3).astype(np.float64)
+    alpha = 1.3
+    ss = _sliceset(S, dev)
+    im = engine.ImageSet(Dm, dev)
+    a = torch.tensor([alpha], dtype=torch.float64, device=dev)
+    r = engine.match_resident(ss, im, precision, K=8, rescore_fp64=True, alpha=a)
+    ref_idx, ref_d = matching.topk_squared(alpha * S.astype(np.float32).astype(np.float64),
+                                           Dm.astype(np.float32).astype(np.float64), 8)
+    best = r["index"].cpu().numpy()
+    assert (best == ref_idx[:, 0]).all()
+    # FP64 rescoring of the candidates matches numpy FP64 distances
+    cand = r["topk_idx"].cpu().numpy()
+    d64 = r["d64"].cpu().numpy()
+    S32 = S.astype(np.float32).astype(np.float64)
+    D32 = Dm.astype(np.float32).astype(np.float64)
+    direct = ((alpha * S32[cand] - D32[:, None, :]) ** 2).sum(-1)
+    np.testing.assert_allclose(d64, direct, rtol=1e-12)
+    np.testing.assert_allclose(r["d32"].cpu().numpy(), direct, rtol=1e-4)
+    if precision == "fp32":
+        # candidate sets equal the exact top-8 except at near-ties of the 8th
+        same = np.mean([len(set(c) & set(e)) / 8 for c, e in zip(cand, ref_idx)])
+        assert same >= 0.99
+        # ranking values on the centred residuals s' = s - c, d' = d / alpha - c:
+        # v = |s'|^2 - 2 s'.d' = |s - d / alpha|^2 - |d / alpha - c|^2
+        c = S32.mean(0)
+        v = r["topk_val"].cpu().numpy()[:, 0]
+        want = ref_d[:, 0] / alpha ** 2 - ((D32 / alpha - c) ** 2).sum(1)
+        np.testing.assert_allclose(v, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+
+
+def test_ties_lowest_index(dev):
+    import torch
+    from spinifel_amd import engine
+    rng = np.random.default_rng(3)
+    base = rng.random((40, 256))
+    S = np.concatenate([base, base, base])        # every slice three times
+    D = base[rng.integers(0, 40, 60)] + 0.01 * rng.random((60, 256))
+    ss = _sliceset(S, dev)
+    im = engine.ImageSet(D, dev)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    r = engine.match_resident(ss, im, "fp32", K=4, rescore_fp64=True, alpha=one)
+    ref = matching.nearest_neighbor(S.astype(np.float32), D.astype(np.float32))
+    assert (r["index"].cpu().numpy() == ref).all()
+    assert (r["index"].cpu().numpy() < 40).all()
+    r1 = engine.match_resident(ss, im, "fp32", K=1, rescore_fp64=False, alpha=one)
+    assert (r1["index"].cpu().numpy() < 40).all()
+
+
+def test_match_m41_dropin_vs_oracle(dev):
+    """The drop-in match on the CPU-runnable workload (M=41, 1x32x32 px,
+    2000 reference orientations, 500 Poisson images): identical argmin to the
+    oracle's restatement of the reference CPU path, in every form."""
+    from spinifel_amd import orientation_matching, settings as st
+    w = workload("m41_1k_px")
+    print("case: drop-in match m41_1k_px", flush=True)
+    ora = matching.slicing_and_match(w.ac, w.images, w.ppr, w.pdr, w.ref_quats,
+                                     return_details=True)
+    s = st.SpinifelSettings(N_orientations=2000, N_batch_size=1000,
+                            reduced_det_shape=w.det_shape, oversampling=1)
+    res = {}
+    for form in [dict(precision="fp32"), dict(precision="fp32", topk=8, rescore=True),
+                 dict(precision="bf16"), dict(precision="fp8")]:
+        r = orientation_matching.slicing_and_match(w.ac, w.images, w.ppr, w.pdr, w.ref_quats,
+                                                   settings=s, return_details=True, **form)
+        key = f"{form['precision']}{'+r' if form.get('rescore') or form['precision'] != 'fp32' else ''}"
+        res[key] = (r["index"] == ora["index"]).mean()
+        np.testing.assert_allclose(r["alpha"][0], ora["alpha"], rtol=1e-5)
+    print(res)
+    for k, v in res.items():
+        assert v == 1.0, (k, v)
+    # the quaternions themselves (the return value of match)
+    q = orientation_matching.slicing_and_match(w.ac, w.images, w.ppr, w.pdr, w.ref_quats,
+                                               settings=s)
+    np.testing.assert_array_equal(q, ora["orientations"])
+
+
+def test_match_edge_cases(dev):
+    from spinifel_amd import orientation_matching, settings as st
+    w = workload("m41_1k_px")
+    s = st.SpinifelSettings(N_orientations=2000, N_batch_size=1000,
+                            reduced_det_shape=w.det_shape, oversampling=1)
+    empty = np.zeros((0,) + w.det_shape, np.float32)
+    out = orientation_matching.slicing_and_match(w.ac, empty, w.ppr, w.pdr, w.ref_quats, settings=s)
+    assert out.shape == (0, 4)
+    with pytest.raises(AssertionError):
+        orientation_matching.slicing_and_match(w.ac, w.images[:, 0], w.ppr, w.pdr, w.ref_quats,
+                                               settings=s)
+    bad = st.SpinifelSettings(N_orientations=2000, N_batch_size=300,
+                              reduced_det_shape=w.det_shape)
+    with pytest.raises(AssertionError):
+        orientation_matching.slicing_and_match(w.ac, w.images, w.ppr, w.pdr, w.ref_quats,
+                                               settings=bad)
+    # one image, ragged orientation count
+    s1 = st.SpinifelSettings(N_orientations=333, N_batch_size=333, reduced_det_shape=w.det_shape)
+    r = orientation_matching.slicing_and_match(w.ac, w.images[:1], w.ppr, w.pdr,
+                                               w.ref_quats[:333], settings=s1, return_details=True)
+    ora = matching.slicing_and_match(w.ac, w.images[:1], w.ppr, w.pdr, w.ref_quats[:333],
+                                     return_details=True)
+    assert r["index"][0] == ora["index"][0]
+    # out-of-domain q-points raise like the reference's assert
+    with pytest.raises(AssertionError):
+        orientation_matching.slicing_and_match(w.ac, w.images, w.ppr, w.pdr / 4.0, w.ref_quats,
+                                               settings=s)
