@@ -1,0 +1,313 @@
+#!/usr/bin/env python3
+"""Orientation-matching benchmark on one MI355X.
+
+One step = one ``match`` of N_i resident images against N_o reference
+orientations of a resident model: fine-grid load of the model, FP64
+rotation + ES-kernel NUFFT slicing of all orientations, std scaling,
+centred residual operands, MFMA cross term with fused per-image top-K,
+merge, and (BF16 / FP8 forms) FP32 + FP64 rescoring of 8 candidates.
+Image statistics are computed once when the images become resident (the
+reference's class path does the same, spinifel/sequential/orientation_matching.py:70-76).
+
+Default workload: M=81 model (800 Gaussian atoms), 4 x 32 x 128 = 16,384
+Ewald-sphere pixels, 10,000 random reference quaternions, 10,000 Poisson
+images (1 photon/px) at random true orientations; synthetic, seeded.
+
+Prints one JSON line.  ``value`` = images/s of the BF16 + rescore form;
+the FP32 (plain) and FP8 forms are timed the same way and reported under
+``forms``.  Accuracy for every form: argmin agreement and relative distance
+error against FP64 distances (numpy) on an image subset; NUFFT slice error
+against the FP64 direct NUDFT on a pixel subsample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"hbm_gbs": 8000.0, "fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}   # GB/s, TFLOP/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="m81_16k_px")
+    p.add_argument("--forms", default="fp32,bf16,fp8")
+    p.add_argument("--n-ref", type=int, default=None)
+    p.add_argument("--n-img", type=int, default=None)
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--eps", type=float, default=1e-6)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-check", action="store_true", help="skip the accuracy checks")
+    p.add_argument("--cpu-seconds", type=float, default=20.0,
+                   help="target CPU time of the baseline sample")
+    return p.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(args, torch, engine, synth):
+    p = dict(synth.WORKLOADS[args.workload])
+    if args.n_ref:
+        p["n_ref"] = args.n_ref
+    if args.n_img:
+        p["n_img"] = args.n_img
+    dev = torch.device("cuda", 0)
+    ac = synth.make_model(p["M"], p["atoms"], args.seed)
+    ppr, pdr = synth.detector(p["panels"], p["rows"], p["cols"])
+    E = float(pdr.max())
+    ref = synth.random_quaternions(p["n_ref"], synth.rng_for(args.seed, synth.STREAM_REF))
+    true = synth.random_quaternions(p["n_img"], synth.rng_for(args.seed, synth.STREAM_TRUE))
+    plan = engine.NufftPlan(p["M"], args.eps, 0)
+    plan.set_model(ac)
+    q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
+    scale = np.pi / E
+    # images: GPU slices at the true orientations, clipped, scaled, Poisson (numpy, seeded)
+    true_sl = plan.slices(torch.as_tensor(true).to(dev), q, scale)["s32"][:, :q.shape[1]]
+    imgs = synth.poisson_images(true_sl.double().cpu().numpy(), p["mean_photons"],
+                                synth.rng_for(args.seed, synth.STREAM_POISSON))
+    del true_sl
+    ac_dev = torch.as_tensor(ac).to(dev)
+    quats_dev = torch.as_tensor(ref).to(dev)
+    return dict(p=p, ac=ac, ac_dev=ac_dev, ppr=ppr, pdr=pdr, E=E, ref=ref, true=true, plan=plan,
+                q=q, scale=scale, quats=quats_dev, images=imgs, dev=dev)
+
+
+def run_form(form, W, args, torch, engine, timers):
+    precision = form
+    K = 1 if form == "fp32" else 8
+    resc = form != "fp32"
+    imgs = W["imgset"]
+    plan = W["plan"]
+
+    def step(t=None):
+        plan.set_model(W["ac_dev"])
+        sl = engine.SliceSet.compute(plan, W["quats"], W["q"], W["scale"], timers=t)
+        return sl, engine.match_resident(sl, imgs, precision, K=K, rescore_fp64=resc, timers=t)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timers.clear()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        sl, r = step(timers)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - s0)
+    total = time.perf_counter() - t0
+    N_i = imgs.N_i
+    out = dict(images_per_s=N_i * args.steps / total, ms_per_step=1e3 * total / args.steps,
+               p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+               K=K, rescore="fp32+fp64" if resc else None)
+    kern = {}
+    for name in list(timers.spans):
+        ms = timers.ms(name)
+        kern[name] = float(np.mean(ms))
+    out["kernel_ms"] = kern
+    return out, sl, r
+
+
+def roofline_entries(W, kern_ms, forms, engine):
+    p = W["p"]
+    N_o, N_i = p["n_ref"], p["n_img"]
+    N_p = int(W["q"].shape[1])
+    P = N_o * N_p
+    npad = W["plan"].padded_edge
+    ent = {}
+    if "nufft" in kern_ms:
+        # FP32 slice write + real fine grid read once + FP64 q + FP64 quats + FP64 row sums
+        byts = 4 * P + 4 * npad ** 3 + 24 * N_p + 32 * N_o + 16 * N_o
+        t = kern_ms["nufft"] * 1e-3
+        ent["nufft"] = dict(bound="hbm", achieved=byts / t / 1e9, peak=PEAK["hbm_gbs"], unit="GB/s",
+                            frac=byts / t / 1e9 / PEAK["hbm_gbs"], traffic=None,
+                            algorithmic_bytes=byts, ms=kern_ms["nufft"],
+                            taps_per_s=P * W["plan"].width ** 3 / t)
+    for f in forms:
+        key = "gemm_" + f
+        if key in kern_ms:
+            flops = 2.0 * N_o * N_i * N_p
+            t = kern_ms[key] * 1e-3
+            ent[key] = dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK[f], unit="TFLOP/s",
+                            frac=flops / t / 1e12 / PEAK[f], traffic=None, algorithmic_flops=flops,
+                            ms=kern_ms[key])
+    return ent
+
+
+def accuracy(form_res, W, sl, torch, n_sub=256):
+    """Argmin agreement and distance error vs FP64 numpy distances on an image subset."""
+    S = sl.f32[:, :sl.N_p].double().cpu().numpy()
+    D = W["images"].reshape(W["images"].shape[0], -1).astype(np.float64)
+    alpha = D.std() / S.std()
+    sub = np.random.default_rng(123).choice(D.shape[0], min(n_sub, D.shape[0]), replace=False)
+    Ds = D[sub]
+    d2 = (alpha * alpha) * (S * S).sum(1)[:, None] - 2 * alpha * (S @ Ds.T) + (Ds * Ds).sum(1)[None, :]
+    ref_idx = np.argmin(d2, axis=0)
+    srt = np.sort(d2, axis=0)
+    margin = (srt[1] - srt[0]) / srt[0]
+    out = {}
+    c = sl.center[:sl.N_p].double().cpu().numpy()
+    for f, r in form_res.items():
+        idx = r["index"].long().cpu().numpy()[sub]
+        agree = float((idx == ref_idx).mean())
+        dref = d2[idx, np.arange(len(sub))]
+        v = r["topk_val"].double().cpu().numpy()[sub, 0]
+        a = float(r["alpha"].item())
+        est = a * a * (v + ((Ds / a - c[None, :]) ** 2).sum(1))
+        e_gemm = np.abs(est - d2[r["topk_idx"].long().cpu().numpy()[sub, 0], np.arange(len(sub))]) \
+            / d2[r["topk_idx"].long().cpu().numpy()[sub, 0], np.arange(len(sub))]
+        ent = dict(argmin_agreement=agree, gemm_rel_dist_err_max=float(e_gemm.max()),
+                   gemm_rel_dist_err_mean=float(e_gemm.mean()))
+        if "best_d64" in r:
+            bd = r["best_d64"].cpu().numpy()[sub]
+            e = np.abs(bd - dref) / dref
+            ent.update(final_rel_dist_err_max=float(e.max()), final_rel_dist_err_mean=float(e.mean()))
+        out[f] = ent
+    out["_subset"] = dict(n_images=int(len(sub)), alpha_fp64=float(alpha),
+                          min_rel_margin=float(margin.min()),
+                          n_margin_below_1e6=int((margin < 1e-6).sum()))
+    return out
+
+
+def nufft_check(W, sl):
+    from oracle import es_nufft, geometry
+    rng = np.random.default_rng(7)
+    o = rng.integers(0, sl.N_o, 200)
+    j = rng.integers(0, sl.N_p, 200)
+    H, K, L = geometry.gen_nonuniform_normalized_positions(W["ref"][o], W["ppr"], W["E"], 1)
+    Np = sl.N_p
+    Hs, Ks, Ls = (H.reshape(200, Np)[np.arange(200), j], K.reshape(200, Np)[np.arange(200), j],
+                  L.reshape(200, Np)[np.arange(200), j])
+    ref = es_nufft.nudft3d2(Hs, Ks, Ls, W["ac"].astype(np.float64)).real
+    got = sl.f32[o, j].double().cpu().numpy()
+    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+
+def cpu_baseline(W, sl, args):
+    """The oracle (FP64 ES-kernel NUFFT at the reference default eps 1e-12, C/OpenMP,
+    plus numpy FP64 distances + argmin) on a bounded sample, scaled to the job."""
+    from oracle import es_nufft, geometry, matching
+    N_o, N_i = W["p"]["n_ref"], W["p"]["n_img"]
+    ppr, E = W["ppr"], W["E"]
+    budget = args.cpu_seconds
+    t = time.perf_counter()
+    b, n, w, beta = es_nufft.fine_grid(W["ac"].astype(np.float64), -1, 1e-12)
+    t_grid = time.perf_counter() - t
+
+    def slice_time(k):
+        H, K, L = geometry.gen_nonuniform_normalized_positions(W["ref"][:k], ppr, E, 1)
+        t0 = time.perf_counter()
+        es_nufft.interp(b, H, K, L, w, beta)
+        return time.perf_counter() - t0
+
+    t1 = slice_time(4)
+    n_s = int(max(4, min(N_o, 0.5 * budget / max(t1 / 4, 1e-9))))
+    ts = slice_time(n_s)
+    S = sl.f32[:, :sl.N_p].double().cpu().numpy()
+    D = W["images"].reshape(N_i, -1).astype(np.float64)
+    t0 = time.perf_counter()
+    matching.nearest_neighbor(S, D[:16])
+    tm1 = time.perf_counter() - t0
+    n_m = int(max(16, min(N_i, 0.5 * budget / max(tm1 / 16, 1e-9))))
+    t0 = time.perf_counter()
+    matching.nearest_neighbor(S, D[:n_m])
+    tm = time.perf_counter() - t0
+    total = t_grid + ts * N_o / n_s + tm * N_i / n_m
+    cores = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return dict(value=N_i / total, unit="images/s", cores=min(cores, threads), kind="port",
+                sample=(f"oracle FP64 ES-NUFFT eps 1e-12 (w={w}) on {n_s} of {N_o} orientations "
+                        f"({ts:.1f} s) + fine grid ({t_grid:.1f} s) + numpy FP64 distances/argmin "
+                        f"for {n_m} of {N_i} images vs all {N_o} ({tm:.1f} s), scaled linearly "
+                        f"to the full job ({total:.0f} s)"),
+                seconds_full_job=total)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from spinifel_amd import engine, synth
+    engine.require_gpu()
+    forms = [f for f in args.forms.split(",") if f]
+    log("building workload", args.workload)
+    W = build_workload(args, torch, engine, synth)
+    W["imgset"] = engine.ImageSet(W["images"].reshape(W["images"].shape[0], -1), W["dev"])
+    torch.cuda.synchronize()
+    res, last, timers = {}, {}, {}
+    sl = None
+    for f in forms:
+        log("timing form", f)
+        t = engine.Timers()
+        out, sl, r = run_form(f, W, args, torch, engine, t)
+        res[f] = out
+        last[f] = r
+        log(f, json.dumps({k: v for k, v in out.items() if k != "kernel_ms"}), out["kernel_ms"])
+    kern_all = {}
+    for f in forms:
+        kern_all.update(res[f]["kernel_ms"])
+    roof = roofline_entries(W, kern_all, forms, engine)
+    if not args.no_check:
+        log("accuracy checks")
+        acc = accuracy(last, W, sl, torch)
+        for f in forms:
+            res[f]["accuracy"] = acc[f]
+        nufft_err = nufft_check(W, sl)
+    else:
+        acc, nufft_err = {}, None
+    head = "bf16" if "bf16" in forms else forms[0]
+    hk = res[head]["kernel_ms"]
+    dom = max([k for k in hk if k in roof], key=lambda k: hk[k])
+    rl = {k: roof[dom][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
+    rl["kernel"] = dom
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(W, sl, args)
+    p = W["p"]
+    line = {
+        "metric": "images/s oriented, 10k orients x 16k px: BF16+rescore vs FP32; p50/p99 latency",
+        "value": res[head]["images_per_s"],
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[head]["ms_per_step"],
+        "p50_ms": res[head]["p50_ms"],
+        "p99_ms": res[head]["p99_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic (seeded Gaussian-atom model, Poisson images)",
+        "config": {"workload": (f"{args.workload}: M={p['M']} model, {p['panels']}x{p['rows']}x"
+                                f"{p['cols']} px, {p['n_ref']} orientations, {p['n_img']} images, "
+                                f"NUFFT eps {args.eps}"),
+                   "form": f"{head}" + (" + top-8 FP32/FP64 rescore" if head != "fp32" else "")},
+        "speedup_vs_fp32": (res[head]["images_per_s"] / res["fp32"]["images_per_s"]
+                            if "fp32" in res and head != "fp32" else None),
+        "forms": res,
+        "roofline": rl,
+        "kernels": roof,
+        "nufft_rel_l2_vs_nudft": nufft_err,
+        "accuracy_subset": acc.get("_subset") if acc else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -198,6 +198,20 @@ int mtip_match_topk(int precision, const void *slices, int64_t ld_s, const float
                     float *top_val, void *stream);
 
 /*
+ * The two halves of mtip_match_topk, for callers that time or reuse them:
+ * mtip_match_partial runs the MFMA GEMM and writes per-(image, 128-row
+ * orientation tile) top-K packed keys to the workspace; mtip_topk_merge
+ * reduces them to the per-image top-K.  Same arguments and rules as
+ * mtip_match_topk.
+ */
+int mtip_match_partial(int precision, const void *slices, int64_t ld_s, const float *slice_scale,
+                       const double *slice_sumsq, int64_t N_o, const void *images, int64_t ld_d,
+                       const float *image_scale, int64_t N_i, int64_t N_p, const double *alpha,
+                       int K, void *workspace, size_t workspace_bytes, void *stream);
+int mtip_topk_merge(const void *workspace, int64_t N_o, int64_t N_i, int K, int32_t *top_idx,
+                    float *top_val, void *stream);
+
+/*
  * Rescore K candidates per image on FP32 slices/images:
  *   d(n, i) = sum_p (alpha s_n[p] - d_i[p])^2
  * once with FP32 arithmetic (d32, nullable) and once in FP64 (d64,

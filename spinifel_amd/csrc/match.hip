@@ -604,51 +604,67 @@ extern "C" size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int
     return (size_t)(N_i * nta * K) * sizeof(uint64_t);
 }
 
-extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
-                               const float *slice_scale, const double *slice_sumsq, int64_t N_o,
-                               const void *images, int64_t ld_d, const float *image_scale,
-                               int64_t N_i, int64_t N_p, const double *alpha, int K,
-                               void *workspace, size_t workspace_bytes, int32_t *top_idx,
-                               float *top_val, void *stream)
+namespace {
+
+int check_match_args(const char *fn, int precision, const void *slices, int64_t ld_s,
+                     const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                     const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                     int64_t N_p, const double *alpha, int K)
 {
+    const std::string f(fn);
     const int eb = prec_bytes(precision);
-    MTIP_REQUIRE(eb > 0, MTIP_ERR_ARG, "mtip_match_topk: unknown precision");
-    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_match_topk: K must be 1, 2, 4 or 8");
-    MTIP_REQUIRE(N_o >= 0 && N_i >= 0 && N_p >= 0, MTIP_ERR_ARG, "mtip_match_topk: negative size");
-    MTIP_REQUIRE(N_o <= 0x7fffffff, MTIP_ERR_ARG, "mtip_match_topk: N_o too large");
+    MTIP_REQUIRE(eb > 0, MTIP_ERR_ARG, f + ": unknown precision");
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, f + ": K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(N_o >= 0 && N_i >= 0 && N_p >= 0, MTIP_ERR_ARG, f + ": negative size");
+    MTIP_REQUIRE(N_o <= 0x7fffffff, MTIP_ERR_ARG, f + ": N_o too large");
     if (N_i == 0) return MTIP_OK;
-    MTIP_REQUIRE(N_o > 0, MTIP_ERR_ARG, "mtip_match_topk: no orientations");
-    MTIP_REQUIRE(N_p > 0 && N_p % 16 == 0, MTIP_ERR_ARG, "mtip_match_topk: N_p must be a positive multiple of 16");
+    MTIP_REQUIRE(N_o > 0, MTIP_ERR_ARG, f + ": no orientations");
+    MTIP_REQUIRE(N_p > 0 && N_p % 16 == 0, MTIP_ERR_ARG, f + ": N_p must be a positive multiple of 16");
     MTIP_REQUIRE(ld_s >= N_p && ld_d >= N_p && (ld_s * eb) % 16 == 0 && (ld_d * eb) % 16 == 0,
-                 MTIP_ERR_ARG, "mtip_match_topk: leading dimensions must be >= N_p and 16-byte multiples");
-    MTIP_REQUIRE(slices && images && slice_sumsq && alpha && top_idx, MTIP_ERR_ARG,
-                 "mtip_match_topk: null pointer");
+                 MTIP_ERR_ARG, f + ": leading dimensions must be >= N_p and 16-byte multiples");
+    MTIP_REQUIRE(slices && images && slice_sumsq && alpha, MTIP_ERR_ARG, f + ": null pointer");
     MTIP_REQUIRE(precision != MTIP_FP8 || (slice_scale && image_scale), MTIP_ERR_ARG,
-                 "mtip_match_topk: FP8 needs per-row scales");
+                 f + ": FP8 needs per-row scales");
     MTIP_REQUIRE(((uintptr_t)slices % 16) == 0 && ((uintptr_t)images % 16) == 0, MTIP_ERR_ARG,
-                 "mtip_match_topk: operands must be 16-byte aligned");
+                 f + ": operands must be 16-byte aligned");
+    const int64_t nta = mtip::ceil_div(N_o, BM), ntb = mtip::ceil_div(N_i, BN);
+    MTIP_REQUIRE(nta * ntb < 0x7fffffff, MTIP_ERR_ARG, f + ": grid too large");
+    return MTIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_s,
+                                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                                  const void *images, int64_t ld_d, const float *image_scale,
+                                  int64_t N_i, int64_t N_p, const double *alpha, int K,
+                                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    int rc = check_match_args("mtip_match_partial", precision, slices, ld_s, slice_scale,
+                              slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K);
+    if (rc != MTIP_OK || N_i == 0) return rc;
     size_t need = mtip_match_workspace_size(N_o, N_i, K, precision);
     MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM,
-                 "mtip_match_topk: workspace too small");
+                 "mtip_match_partial: workspace too small");
+    const int eb = prec_bytes(precision);
     const int nta = (int)mtip::ceil_div(N_o, BM);
-    const int64_t ntb = mtip::ceil_div(N_i, BN);
-    MTIP_REQUIRE((int64_t)nta * ntb < 0x7fffffff, MTIP_ERR_ARG, "mtip_match_topk: grid too large");
+    const int ntb = (int)mtip::ceil_div(N_i, BN);
     hipStream_t st = (hipStream_t)stream;
     uint64_t *part = (uint64_t *)workspace;
     const int64_t kbytes = N_p * eb;
 #define MTIP_GEMM_CASE(P)                                                                        \
     switch (K) {                                                                                 \
     case 1: launch_gemm_topk<P, 1>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
                                    part, st); break;                                             \
     case 2: launch_gemm_topk<P, 2>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
                                    part, st); break;                                             \
     case 4: launch_gemm_topk<P, 4>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb,  \
+                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
                                    part, st); break;                                             \
     default: launch_gemm_topk<P, 8>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,    \
-                                    slice_scale, image_scale, slice_sumsq, alpha, nta, (int)ntb, \
+                                    slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,      \
                                     part, st); break;                                            \
     }
     if (precision == MTIP_FP32) { MTIP_GEMM_CASE(MTIP_FP32) }
@@ -656,7 +672,19 @@ extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
     else { MTIP_GEMM_CASE(MTIP_FP8) }
 #undef MTIP_GEMM_CASE
     MTIP_HIP_CHECK(hipGetLastError());
-    const int64_t nkeys = (int64_t)nta * K;
+    return MTIP_OK;
+}
+
+extern "C" int mtip_topk_merge(const void *workspace, int64_t N_o, int64_t N_i, int K,
+                               int32_t *top_idx, float *top_val, void *stream)
+{
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_topk_merge: K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(N_o > 0 && N_i >= 0, MTIP_ERR_ARG, "mtip_topk_merge: bad size");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(workspace && top_idx, MTIP_ERR_ARG, "mtip_topk_merge: null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t *part = (const uint64_t *)workspace;
+    const int64_t nkeys = mtip::ceil_div(N_o, BM) * K;
     switch (K) {
     case 1: launch_topk_rows<1>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
     case 2: launch_topk_rows<2>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
@@ -665,6 +693,21 @@ extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
     }
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
+}
+
+extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
+                               const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                               const void *images, int64_t ld_d, const float *image_scale,
+                               int64_t N_i, int64_t N_p, const double *alpha, int K,
+                               void *workspace, size_t workspace_bytes, int32_t *top_idx,
+                               float *top_val, void *stream)
+{
+    MTIP_REQUIRE(top_idx || N_i == 0, MTIP_ERR_ARG, "mtip_match_topk: null top_idx");
+    int rc = mtip_match_partial(precision, slices, ld_s, slice_scale, slice_sumsq, N_o, images,
+                                ld_d, image_scale, N_i, N_p, alpha, K, workspace, workspace_bytes,
+                                stream);
+    if (rc != MTIP_OK || N_i == 0) return rc;
+    return mtip_topk_merge(workspace, N_o, N_i, K, top_idx, top_val, stream);
 }
 
 extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,

@@ -45,6 +45,52 @@ def padded(n, mult=_PAD):
     return ((int(n) + mult - 1) // mult) * mult
 
 
+class Timers:
+    """HIP-event spans on the current stream, keyed by name.
+
+    ``with timers.span("nufft"): ...`` records an event pair around the
+    enclosed launches; ``ms(name)`` (after a synchronize) lists durations.
+    """
+
+    def __init__(self):
+        self.spans = {}
+
+    def span(self, name):
+        timers = self
+
+        class _Span:
+            def __enter__(self_):
+                self_.a = torch.cuda.Event(enable_timing=True)
+                self_.b = torch.cuda.Event(enable_timing=True)
+                self_.a.record()
+                return self_
+
+            def __exit__(self_, *exc):
+                self_.b.record()
+                timers.spans.setdefault(name, []).append((self_.a, self_.b))
+                return False
+
+        return _Span()
+
+    def ms(self, name):
+        return [a.elapsed_time(b) for a, b in self.spans.get(name, [])]
+
+    def clear(self):
+        self.spans = {}
+
+
+class _NoSpan:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+def _span(timers, name):
+    return timers.span(name) if timers is not None else _NoSpan()
+
+
 class NufftPlan:
     """Type-2 NUFFT plan of a cubic M^3 model (ES kernel, sigma = 2)."""
 
@@ -108,7 +154,8 @@ class NufftPlan:
             raise AssertionError(f"{int(bad.item())} NU points outside [-3pi, 3pi)")
         return out
 
-    def slices(self, quats, q, scale, out=None, ld=None, want_bf16=False, bad=None, stream=None):
+    def slices(self, quats, q, scale, out=None, ld=None, want_bf16=False, bad=None, stream=None,
+               timers=None):
         """Slices at N_o orientations for pixel q-vectors q (3, N_p).
 
         quats: (N_o, 4) float64 device tensor; q: (3, N_p) float64 device tensor.
@@ -129,9 +176,11 @@ class NufftPlan:
         rq = torch.empty(N_o, dtype=torch.float64, device=self.device)
         if bad is None:
             bad = torch.zeros(1, dtype=torch.int32, device=self.device)
-        _lib.check(self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, float(scale),
-                                             _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
-                                             _stream(stream)), "mtip_plan_slices")
+        with _span(timers, "nufft"):
+            rc = self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, float(scale),
+                                           _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
+                                           _stream(stream))
+        _lib.check(rc, "mtip_plan_slices")
         return dict(s32=out, bf16=bf, row_sum=rs, row_sumsq=rq, bad=bad, N_p=N_p, ld=ld)
 
 
@@ -249,7 +298,8 @@ def workspace_bytes(N_o, N_i, K, precision):
     return int(_lib.load().mtip_match_workspace_size(N_o, N_i, K, _lib.PRECISIONS[precision]))
 
 
-def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stream=None):
+def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stream=None,
+         timers=None):
     """Top-K orientations per image by v = alpha^2 ||s||^2 - 2 alpha s.d.
 
     S, D: Operand; returns (idx int32 (N_i, K), val float32 (N_i, K)).
@@ -261,15 +311,20 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
         workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
     idx = torch.empty((N_i, K), dtype=torch.int32, device=dev)
     val = torch.empty((N_i, K), dtype=torch.float32, device=dev)
-    _lib.check(lib.mtip_match_topk(_lib.PRECISIONS[precision], _ptr(S.data), S.data.shape[1],
-                                   _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
-                                   D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
-                                   _ptr(workspace), workspace.numel(), _ptr(idx), _ptr(val),
-                                   _stream(stream)), "mtip_match_topk")
+    st = _stream(stream)
+    with _span(timers, "gemm_" + precision):
+        rc = lib.mtip_match_partial(_lib.PRECISIONS[precision], _ptr(S.data), S.data.shape[1],
+                                    _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
+                                    D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
+                                    _ptr(workspace), workspace.numel(), st)
+    _lib.check(rc, "mtip_match_partial")
+    with _span(timers, "merge"):
+        rc = lib.mtip_topk_merge(_ptr(workspace), N_o, N_i, K, _ptr(idx), _ptr(val), st)
+    _lib.check(rc, "mtip_topk_merge")
     return idx, val
 
 
-def rescore(S32, D32, N_p, alpha, cand, stream=None):
+def rescore(S32, D32, N_p, alpha, cand, stream=None, timers=None):
     """FP32 and FP64 distances of the candidates; FP64 argmin per image."""
     lib = _lib.load()
     N_i, K = cand.shape
@@ -278,9 +333,11 @@ def rescore(S32, D32, N_p, alpha, cand, stream=None):
     bd = torch.empty(N_i, dtype=torch.float64, device=dev)
     d32 = torch.empty((N_i, K), dtype=torch.float32, device=dev)
     d64 = torch.empty((N_i, K), dtype=torch.float64, device=dev)
-    _lib.check(lib.mtip_rescore(_ptr(S32), S32.shape[1], S32.shape[0], _ptr(D32), D32.shape[1], N_i, N_p,
-                                _ptr(alpha), _ptr(cand.contiguous()), K, _ptr(best), _ptr(bd),
-                                _ptr(d32), _ptr(d64), _stream(stream)), "mtip_rescore")
+    with _span(timers, "rescore"):
+        rc = lib.mtip_rescore(_ptr(S32), S32.shape[1], S32.shape[0], _ptr(D32), D32.shape[1], N_i,
+                              N_p, _ptr(alpha), _ptr(cand.contiguous()), K, _ptr(best), _ptr(bd),
+                              _ptr(d32), _ptr(d64), _stream(stream))
+    _lib.check(rc, "mtip_rescore")
     return dict(best=best, best_d64=bd, d32=d32, d64=d64)
 
 
@@ -325,14 +382,14 @@ class SliceSet:
         self._center = None
 
     @classmethod
-    def compute(cls, plan, quats, q, scale, precision="fp32", stream=None):
+    def compute(cls, plan, quats, q, scale, precision="fp32", stream=None, timers=None):
         """NUFFT slices of the plan's model at quaternions `quats` (N_o, 4)."""
         dev = plan.device
         quats = torch.as_tensor(np.ascontiguousarray(quats, np.float64)).to(dev) \
             if isinstance(quats, np.ndarray) else quats.to(dev, torch.float64)
         q = torch.as_tensor(np.ascontiguousarray(q, np.float64)).to(dev) \
             if isinstance(q, np.ndarray) else q.to(dev, torch.float64)
-        r = plan.slices(quats, q, scale, stream=stream)
+        r = plan.slices(quats, q, scale, stream=stream, timers=timers)
         return cls(r["s32"], r["N_p"], r["row_sum"], r["row_sumsq"], r["bf16"], quats, r["bad"])
 
     @classmethod
@@ -368,7 +425,7 @@ def _one(dev):
 
 
 def match_resident(slices, images, precision="fp32", K=None, rescore_fp64=None, alpha=None,
-                   center=True, workspace=None, stream=None):
+                   center=True, workspace=None, stream=None, timers=None):
     """Match resident images against resident slices.
 
     Ranking uses the residuals s' = s - c and d' = d / alpha - c with c the
@@ -388,14 +445,15 @@ def match_resident(slices, images, precision="fp32", K=None, rescore_fp64=None, 
     if alpha is None:
         alpha = scaling_ratio(images.row_sum, images.row_sumsq, slices.row_sum, slices.row_sumsq,
                               images.N_p, stream)
-    S = slices.operand(precision, center, stream)
-    D = images.operand(precision, alpha, slices.center if center else None, stream)
+    with _span(timers, "residual"):
+        S = slices.operand(precision, center, stream)
+        D = images.operand(precision, alpha, slices.center if center else None, stream)
     one = _one(alpha.device)
     idx, val = topk(precision, S, S.sumsq, slices.N_o, D, images.N_i, images.N_p, one,
-                    K, workspace, stream)
+                    K, workspace, stream, timers)
     out = dict(topk_idx=idx, topk_val=val, alpha=alpha)
     if rescore_fp64:
-        r = rescore(slices.f32, images.f32, images.N_p, alpha, idx, stream)
+        r = rescore(slices.f32, images.f32, images.N_p, alpha, idx, stream, timers)
         out.update(index=r["best"], best_d64=r["best_d64"], d32=r["d32"], d64=r["d64"])
     else:
         out["index"] = idx[:, 0]
