@@ -72,14 +72,15 @@ def build_workload(args, torch, engine, synth):
     q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
     scale = np.pi / E
     # images: GPU slices at the true orientations, clipped, scaled, Poisson (numpy, seeded)
-    true_sl = plan.slices(torch.as_tensor(true).to(dev), q, scale)["s32"][:, :q.shape[1]]
+    order = torch.as_tensor(engine.tile_order(ppr.shape[1:])).to(dev)
+    true_sl = plan.slices(torch.as_tensor(true).to(dev), q, scale, order=order)["s32"][:, :q.shape[1]]
     imgs = synth.poisson_images(true_sl.double().cpu().numpy(), p["mean_photons"],
                                 synth.rng_for(args.seed, synth.STREAM_POISSON))
     del true_sl
     ac_dev = torch.as_tensor(ac).to(dev)
     quats_dev = torch.as_tensor(ref).to(dev)
     return dict(p=p, ac=ac, ac_dev=ac_dev, ppr=ppr, pdr=pdr, E=E, ref=ref, true=true, plan=plan,
-                q=q, scale=scale, quats=quats_dev, images=imgs, dev=dev)
+                q=q, scale=scale, quats=quats_dev, images=imgs, dev=dev, order=order)
 
 
 def run_form(form, W, args, torch, engine, timers):
@@ -91,7 +92,7 @@ def run_form(form, W, args, torch, engine, timers):
 
     def step(t=None):
         plan.set_model(W["ac_dev"])
-        sl = engine.SliceSet.compute(plan, W["quats"], W["q"], W["scale"], timers=t)
+        sl = engine.SliceSet.compute(plan, W["quats"], W["q"], W["scale"], timers=t, order=W["order"])
         return sl, engine.match_resident(sl, imgs, precision, K=K, rescore_fp64=resc, timers=t)
 
     for _ in range(args.warmup):

@@ -108,12 +108,19 @@ int mtip_plan_execute(mtip_plan *plan, const double *x, const double *y, const d
  * patterns of the same values; row_sum / row_sumsq: device FP64 (N_o)
  * sums of s and s^2 per orientation (for the std scaling :245-248 and the
  * squared norms of the distance expansion).  bad_count as for execute.
+ * pixel_order (nullable, device int32, n_order entries, n_order a multiple
+ * of 256): the order in which pixels are visited, each run of 256 entries
+ * one tile of neighbouring pixels (e.g. 16 x 16 detector blocks), entries -1
+ * for padding; every pixel in [0, N_p) must appear exactly once.  Tiles of
+ * neighbouring pixels let the kernel stage one small box of the fine grid
+ * per tile in LDS.  NULL visits pixels in index order, 256 at a time.
  * Replaces the batched loop of spinifel/sequential/orientation_matching.py:230-241
  * and gen_model_slices (spinifel/slicing.py:10-45).
  */
 int mtip_plan_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
-                     int64_t N_p, double scale, float *out, int64_t ld_out, uint16_t *out_bf16,
-                     double *row_sum, double *row_sumsq, int *bad_count, void *stream);
+                     int64_t N_p, const int32_t *pixel_order, int64_t n_order, double scale,
+                     float *out, int64_t ld_out, uint16_t *out_bf16, double *row_sum,
+                     double *row_sumsq, int *bad_count, void *stream);
 
 /*
  * Rotated, normalised q-points only: H/K/L (device FP64, N_o * N_p,

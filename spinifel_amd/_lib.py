@@ -39,8 +39,8 @@ _SIGS = {
                               _c.POINTER(_i32)]),
     "mtip_plan_set_model": (_i32, [_vp, _vp, _vp]),
     "mtip_plan_execute": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
-    "mtip_plan_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _f64, _vp, _i64, _vp, _vp, _vp,
-                                _vp, _vp]),
+    "mtip_plan_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _f64, _vp, _i64, _vp, _vp,
+                                _vp, _vp, _vp]),
     "mtip_rotate_positions": (_i32, [_vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "mtip_scaling_ratio": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),

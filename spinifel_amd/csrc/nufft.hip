@@ -174,58 +174,63 @@ __global__ __launch_bounds__(256) void k_dft_pass(const float *__restrict__ ac,
 }
 
 // ------------------------------------------------------------ interpolation
+//
+// Lane layout: a group of G lanes (G = 4, 8 or 16 >= w) interpolates one
+// point; lane c of the group owns tap c along the grid's fast axis (L), so
+// every row read of the w x w x w window is one contiguous G-float segment
+// shared by the group, fetched with a buffer load whose row offset
+// (a * np^2 + b * np) is a wave-uniform SGPR.  Per 64-point chunk, each lane
+// first computes the geometry of one point (FP64 rotation, window start,
+// FP32 fractions) into LDS; the wave then walks the chunk G points... 64/G
+// points at a time.  Lane c evaluates the three kernel weights at offset c;
+// the group shares them through LDS; the group sums its lanes' partial
+// results with xor-shuffles.
 
 struct GridView {
     const float *__restrict__ g;
     int np, off, n;
-    float beta, c2;    // c2 = 4 / w^2
+    uint32_t bytes;       // size of the padded grid in bytes (buffer range)
+    float beta_l2e, c2;   // beta * log2(e), 4 / w^2
+    double inv_n;
 };
 
-// phi(z) for the ES kernel; 0 outside the support.
-__device__ __forceinline__ float es_phi(float z, float beta, float c2)
+// phi(z) = exp(beta (sqrt(1 - c2 z^2) - 1)) for c2 z^2 < 1, else 0.
+__device__ __forceinline__ float es_phi(float z, float beta_l2e, float c2)
 {
+    // hardware sqrt / exp2 (about 1 ulp): far below the kernel's own error
     float v = 1.0f - c2 * z * z;
-    return v > 0.0f ? expf(beta * (sqrtf(v) - 1.0f)) : 0.0f;
+    return v > 0.0f ? __builtin_amdgcn_exp2f(beta_l2e * (__builtin_amdgcn_sqrtf(v) - 1.0f)) : 0.0f;
 }
 
-// Interpolate Re c at grid coordinates X (grid units, already checked finite
-// and |X| < 1.5 n).  Wraps into [-n/2, n/2), clamps the window start so every
-// access stays inside the padded grid.
+// Window corner (flat element index into the padded grid) and fractional
+// offsets of a point at grid coordinates X (grid units).  Returns false for
+// points outside |x| < 3 pi (|X| < 1.5 n) or not finite.  The coordinate is
+// wrapped into [-n/2, n/2) and the window start clamped so that every access
+// stays inside the padded grid whatever the rounding.
 template <int W>
-__device__ __forceinline__ float interp_point(const GridView &gv, double X0, double X1, double X2)
+__device__ __forceinline__ bool point_geometry(const GridView &gv, double X0, double X1, double X2,
+                                               int &base, float &t0, float &t1, float &t2)
 {
+    const double lim = 1.5 * gv.n;
+    if (!(fabs(X0) < lim && fabs(X1) < lim && fabs(X2) < lim)) return false;
     const double dn = (double)gv.n;
     double X[3] = {X0, X1, X2};
-    int i0[3];
-    float ker[3][W];
+    int i[3];
+    float t[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double x = X[d] - dn * floor(X[d] / dn + 0.5);
+        double x = X[d] - dn * floor(X[d] * gv.inv_n + 0.5);
         int s = (int)ceil(x - 0.5 * W);
         s = max(s, -gv.n / 2 - W);
         s = min(s, gv.n / 2 + 1);
-        i0[d] = s + gv.off;
-        float t = (float)(x - (double)s);
-#pragma unroll
-        for (int a = 0; a < W; ++a) ker[d][a] = es_phi((float)a - t, gv.beta, gv.c2);
+        i[d] = s + gv.off;
+        t[d] = (float)(x - (double)s);
     }
-    const int64_t np = gv.np;
-    float acc = 0.0f;
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-        const float *plane = gv.g + (int64_t)(i0[0] + a) * np * np;
-        float acc_b = 0.0f;
-#pragma unroll
-        for (int b = 0; b < W; ++b) {
-            const float *row = plane + (int64_t)(i0[1] + b) * np + i0[2];
-            float s = 0.0f;
-#pragma unroll
-            for (int c = 0; c < W; ++c) s = fmaf(ker[2][c], row[c], s);
-            acc_b = fmaf(ker[1][b], s, acc_b);
-        }
-        acc = fmaf(ker[0][a], acc_b, acc);
-    }
-    return acc;
+    base = (i[0] * gv.np + i[1]) * gv.np + i[2];
+    t0 = t[0];
+    t1 = t[1];
+    t2 = t[2];
+    return true;
 }
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
@@ -236,23 +241,10 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
     return (uint16_t)(u >> 16);
 }
 
-template <int W>
-__global__ __launch_bounds__(256) void k_execute(GridView gv, const double *__restrict__ x,
-                                                 const double *__restrict__ y,
-                                                 const double *__restrict__ z, int64_t N,
-                                                 float *__restrict__ out, int *bad)
+__device__ __forceinline__ void wave_sync()
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    const double g = (double)gv.n / (2.0 * M_PI);
-    double X0 = x[j] * g, X1 = y[j] * g, X2 = z[j] * g;
-    const double lim = 1.5 * gv.n;
-    if (!(fabs(X0) < lim && fabs(X1) < lim && fabs(X2) < lim)) {
-        out[j] = __builtin_nanf("");
-        if (bad) atomicAdd(bad, 1);
-        return;
-    }
-    out[j] = interp_point<W>(gv, X0, X1, X2);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Inverse of skopi's rotation of quaternion [w, x, y, z] (eval/geom.py:26-46),
@@ -301,40 +293,387 @@ __device__ __forceinline__ double block_sum_f64(double v, double *s_red)
     return t;
 }
 
-// One workgroup per orientation: rotate every pixel q-vector in FP64, split
-// into window start + FP32 fraction, interpolate, write FP32 (+ bf16) and the
-// orientation's FP64 sum / sum of squares.
-template <int W>
-__global__ __launch_bounds__(256) void k_slices(GridView gv, const double *__restrict__ quats,
-                                                const double *__restrict__ q, int64_t N_p,
+enum { GEOM_ROTATE = 0, GEOM_POINTS = 1 };
+
+// GEOM_ROTATE: one workgroup per orientation (blockIdx.x), points are the
+// N_p pixel q-vectors q (3, N_p) rotated by inv(R(quat)) and scaled by
+// gscale into grid units; out row o, plus the row's FP64 sum / sum of squares.
+// GEOM_POINTS: 256 points per workgroup from x, y, z (radians), gscale = n / 2pi.
+template <int W, int G, int GEOM>
+__global__ __launch_bounds__(256) void k_interp(GridView gv, const double *__restrict__ quats,
+                                                const double *__restrict__ qx,
+                                                const double *__restrict__ qy,
+                                                const double *__restrict__ qz, int64_t N_p,
                                                 double gscale, float *__restrict__ out,
                                                 int64_t ld_out, uint16_t *__restrict__ out_bf16,
                                                 double *__restrict__ row_sum,
                                                 double *__restrict__ row_sumsq, int *bad)
 {
+    constexpr int PPW = 64 / G;    // points per wave per step
+    __shared__ int s_base[4][64];
+    __shared__ float4 s_t[4][64];
+    __shared__ float s_w[4][PPW][2][G];
     __shared__ double s_red[8];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / G, c = lane % G;
+    const int64_t o = (GEOM == GEOM_ROTATE) ? (int64_t)blockIdx.x : 0;
+    double ri[3][3];
+    if (GEOM == GEOM_ROTATE) quat_inv_rot(quats + 4 * o, ri);
+    const int64_t p0 = (GEOM == GEOM_ROTATE) ? 0 : (int64_t)blockIdx.x * 256;
+    const int64_t p1 = (GEOM == GEOM_ROTATE) ? N_p : (p0 + 256 < N_p ? p0 + 256 : N_p);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)gv.g, (short)0, (int)gv.bytes, 0x00020000);
+    const int np = gv.np;
+    float *orow = out + o * ld_out;
+    uint16_t *brow = out_bf16 ? out_bf16 + o * ld_out : nullptr;
+    double sum = 0.0, sumsq = 0.0;
+    int nbad = 0;
+
+    for (int64_t cb = p0 + wave * 64; cb < p1; cb += 256) {
+        // geometry of point cb + lane
+        {
+            const int64_t j = cb + lane;
+            int base = -1;
+            float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+            if (j < p1) {
+                double X0, X1, X2;
+                if (GEOM == GEOM_ROTATE) {
+                    double x = qx[j], y = qx[N_p + j], z = qx[2 * N_p + j];
+                    X0 = (ri[0][0] * x + ri[0][1] * y + ri[0][2] * z) * gscale;
+                    X1 = (ri[1][0] * x + ri[1][1] * y + ri[1][2] * z) * gscale;
+                    X2 = (ri[2][0] * x + ri[2][1] * y + ri[2][2] * z) * gscale;
+                } else {
+                    X0 = qx[j] * gscale;
+                    X1 = qy[j] * gscale;
+                    X2 = qz[j] * gscale;
+                }
+                if (!point_geometry<W>(gv, X0, X1, X2, base, t0, t1, t2)) base = -2;
+            }
+            s_base[wave][lane] = base;
+            s_t[wave][lane] = make_float4(t0, t1, t2, 0.f);
+        }
+        wave_sync();
+#pragma unroll 1
+        for (int it = 0; it < G; ++it) {
+            const int pl = it * PPW + g;
+            const int pb = s_base[wave][pl];
+            const float4 tt = s_t[wave][pl];
+            float wa_c = 0.f, wb_c = 0.f, wc_c = 0.f;
+            if (c < W) {
+                wa_c = es_phi((float)c - tt.x, gv.beta_l2e, gv.c2);
+                wb_c = es_phi((float)c - tt.y, gv.beta_l2e, gv.c2);
+                wc_c = es_phi((float)c - tt.z, gv.beta_l2e, gv.c2);
+            }
+            s_w[wave][g][0][c] = wa_c;
+            s_w[wave][g][1][c] = wb_c;
+            wave_sync();
+            float wa[W], wb[W];
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                wa[a] = s_w[wave][g][0][a];
+                wb[a] = s_w[wave][g][1][a];
+            }
+            const int voff = ((pb >= 0 ? pb : 0) + c) * 4;
+            // issue all W*W row loads first so their latencies overlap
+            uint32_t raw[W][W];
+#pragma unroll
+            for (int a = 0; a < W; ++a)
+#pragma unroll
+                for (int b = 0; b < W; ++b)
+                    raw[a][b] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff,
+                                                                     (a * np * np + b * np) * 4, 0);
+            float acc = 0.f;
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                float tab = 0.f;
+#pragma unroll
+                for (int b = 0; b < W; ++b) tab = fmaf(wb[b], __uint_as_float(raw[a][b]), tab);
+                acc = fmaf(wa[a], tab, acc);
+            }
+            float r = acc * wc_c;
+#pragma unroll
+            for (int m = G / 2; m >= 1; m >>= 1) r += __shfl_xor(r, m, 64);
+            if (c == 0 && pb != -1) {
+                const int64_t j = cb + pl;
+                float v = r;
+                if (pb == -2) {
+                    v = __builtin_nanf("");
+                    ++nbad;
+                }
+                orow[j] = v;
+                if (brow) brow[j] = f32_to_bf16_rne(v);
+                sum += (double)v;
+                sumsq += (double)v * (double)v;
+            }
+            wave_sync();
+        }
+    }
+    if (nbad && bad) atomicAdd(bad, nbad);
+    if (GEOM == GEOM_ROTATE && (row_sum || row_sumsq)) {
+        double ts = block_sum_f64(sum, s_red);
+        double tq = block_sum_f64(sumsq, s_red);
+        if (threadIdx.x == 0) {
+            if (row_sum) row_sum[o] = ts;
+            if (row_sumsq) row_sumsq[o] = tq;
+        }
+    }
+}
+
+// ------------------------------------------------------------ LDS-staged slicing
+//
+// One workgroup per orientation; each wave takes 64-pixel tiles (8 x 8
+// detector pixels when the caller passes a tile order).  A tile's points lie
+// on a small patch of the rotated Ewald sphere, so the union of their w^3
+// windows fits a small box of the fine grid (about 2k floats for the 8 x 8
+// tiles of a 128 x 128 detector at 0.9 Nyquist): the wave stages that box in
+// LDS with coalesced row loads (several box rows per instruction), then runs
+// the taps from LDS, lane c of each G-lane group owning tap c of the fast
+// axis, with compile-time row offsets.  A tile whose box exceeds the per-wave
+// LDS capacity uses direct buffer loads from the grid instead.
+
+constexpr int kBoxCap = 10240;    // floats of LDS per workgroup for the staged box (40 KiB)
+constexpr int kTile = 256;        // pixels per workgroup tile (16 x 16 detector block)
+
+template <int W, int G, int BZP>
+__device__ __forceinline__ float taps_lds(const float *__restrict__ box, int lbase, int by,
+                                          const float (&wa)[W], const float (&wb)[W])
+{
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        const float *pa = box + lbase + a * by * BZP;
+        float tab = 0.f;
+#pragma unroll
+        for (int b = 0; b < W; ++b) tab = fmaf(wb[b], pa[b * BZP], tab);
+        acc = fmaf(wa[a], tab, acc);
+    }
+    return acc;
+}
+
+template <int W>
+__device__ __forceinline__ float taps_direct(__amdgpu_buffer_rsrc_t rsrc, int np, int voff,
+                                             const float (&wa)[W], const float (&wb)[W])
+{
+    uint32_t raw[W][W];
+#pragma unroll
+    for (int a = 0; a < W; ++a)
+#pragma unroll
+        for (int b = 0; b < W; ++b)
+            raw[a][b] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (a * np * np + b * np) * 4, 0);
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        float tab = 0.f;
+#pragma unroll
+        for (int b = 0; b < W; ++b) tab = fmaf(wb[b], __uint_as_float(raw[a][b]), tab);
+        acc = fmaf(wa[a], tab, acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Window corner as separate integer coordinates (padded-grid indices).
+template <int W>
+__device__ __forceinline__ bool point_corner(const GridView &gv, double X0, double X1, double X2,
+                                             int (&ijk)[3], float (&t)[3])
+{
+    const double lim = 1.5 * gv.n;
+    if (!(fabs(X0) < lim && fabs(X1) < lim && fabs(X2) < lim)) return false;
+    const double dn = (double)gv.n;
+    double X[3] = {X0, X1, X2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double x = X[d] - dn * floor(X[d] * gv.inv_n + 0.5);
+        int s = (int)ceil(x - 0.5 * W);
+        s = max(s, -gv.n / 2 - W);
+        s = min(s, gv.n / 2 + 1);
+        ijk[d] = s + gv.off;
+        t[d] = (float)(x - (double)s);
+    }
+    return true;
+}
+
+// Odd LDS row strides the tap loop is compiled for.
+#define MTIP_STRIDE_CASES(X) X(17) X(21) X(25) X(29) X(33) X(41) X(49) X(57) X(65)
+
+__device__ __forceinline__ int odd_stride(int bz)
+{
+    return bz <= 17 ? 17 : bz <= 21 ? 21 : bz <= 25 ? 25 : bz <= 29 ? 29 : bz <= 33 ? 33
+         : bz <= 41 ? 41 : bz <= 49 ? 49 : bz <= 57 ? 57 : bz <= 65 ? 65 : 0;
+}
+
+template <int W, int G>
+__global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *__restrict__ quats,
+                                                    const double *__restrict__ q, int64_t N_p,
+                                                    const int32_t *__restrict__ order,
+                                                    int64_t n_order, double gscale,
+                                                    float *__restrict__ out, int64_t ld_out,
+                                                    uint16_t *__restrict__ out_bf16,
+                                                    double *__restrict__ row_sum,
+                                                    double *__restrict__ row_sumsq, int *bad)
+{
+    constexpr int PPW = 64 / G;
+    __shared__ float s_box[kBoxCap];
+    __shared__ int4 s_ijk[kTile];      // window corner (padded grid) + pixel index / flag
+    __shared__ float4 s_t[kTile];
+    __shared__ float s_w[4][PPW][2][G];
+    __shared__ int s_red_i[4][6];
+    __shared__ double s_red[8];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane / G, c = lane % G;
     const int64_t o = blockIdx.x;
     double ri[3][3];
     quat_inv_rot(quats + 4 * o, ri);
-    const double lim = 1.5 * gv.n;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)gv.g, (short)0, (int)gv.bytes, 0x00020000);
+    const int np = gv.np;
+    float *orow = out + o * ld_out;
+    uint16_t *brow = out_bf16 ? out_bf16 + o * ld_out : nullptr;
     double sum = 0.0, sumsq = 0.0;
     int nbad = 0;
-    for (int64_t j = threadIdx.x; j < N_p; j += blockDim.x) {
-        double qx = q[j], qy = q[N_p + j], qz = q[2 * N_p + j];
-        double X0 = (ri[0][0] * qx + ri[0][1] * qy + ri[0][2] * qz) * gscale;
-        double X1 = (ri[1][0] * qx + ri[1][1] * qy + ri[1][2] * qz) * gscale;
-        double X2 = (ri[2][0] * qx + ri[2][1] * qy + ri[2][2] * qz) * gscale;
-        float v;
-        if (fabs(X0) < lim && fabs(X1) < lim && fabs(X2) < lim) {
-            v = interp_point<W>(gv, X0, X1, X2);
-        } else {
-            v = __builtin_nanf("");
-            ++nbad;
+    const int64_t n_slots = order ? n_order : mtip::ceil_div(N_p, kTile) * kTile;
+
+    for (int64_t tb = 0; tb < n_slots; tb += kTile) {
+        // ---- geometry of this thread's point
+        int pix = -1, ok = 0;
+        int ijk[3] = {0, 0, 0};
+        float t[3] = {0.f, 0.f, 0.f};
+        {
+            const int64_t k = tb + tid;
+            if (order) pix = (k < n_order) ? order[k] : -1;
+            else pix = (k < N_p) ? (int)k : -1;
+            if (pix >= 0) {
+                double x = q[pix], y = q[N_p + pix], z = q[2 * N_p + pix];
+                double X0 = (ri[0][0] * x + ri[0][1] * y + ri[0][2] * z) * gscale;
+                double X1 = (ri[1][0] * x + ri[1][1] * y + ri[1][2] * z) * gscale;
+                double X2 = (ri[2][0] * x + ri[2][1] * y + ri[2][2] * z) * gscale;
+                ok = point_corner<W>(gv, X0, X1, X2, ijk, t) ? 1 : 0;
+            }
         }
-        out[o * ld_out + j] = v;
-        if (out_bf16) out_bf16[o * ld_out + j] = f32_to_bf16_rne(v);
-        sum += (double)v;
-        sumsq += (double)v * (double)v;
+        s_ijk[tid] = make_int4(ijk[0], ijk[1], ijk[2], ok ? pix : (pix >= 0 ? -2 : -1));
+        s_t[tid] = make_float4(t[0], t[1], t[2], 0.f);
+        // ---- box of the tile's windows (block min / max)
+        {
+            int v[6] = {ok ? ijk[0] : 0x7fffffff, ok ? ijk[1] : 0x7fffffff, ok ? ijk[2] : 0x7fffffff,
+                        ok ? -ijk[0] : 0x7fffffff, ok ? -ijk[1] : 0x7fffffff, ok ? -ijk[2] : 0x7fffffff};
+#pragma unroll
+            for (int e = 0; e < 6; ++e) v[e] = wave_min_i(v[e]);
+            if (lane == 0)
+#pragma unroll
+                for (int e = 0; e < 6; ++e) s_red_i[wave][e] = v[e];
+        }
+        __syncthreads();
+        int lo[3], hi[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            lo[e] = min(min(s_red_i[0][e], s_red_i[1][e]), min(s_red_i[2][e], s_red_i[3][e]));
+            hi[e] = -min(min(s_red_i[0][e + 3], s_red_i[1][e + 3]),
+                         min(s_red_i[2][e + 3], s_red_i[3][e + 3]));
+        }
+        const bool any = lo[0] != 0x7fffffff;
+        const int bx = any ? hi[0] - lo[0] + W : 0, by = any ? hi[1] - lo[1] + W : 0;
+        const int bz = any ? hi[2] - lo[2] + G : 0;
+        const int S = odd_stride(bz);
+        const int lpr = bz <= 16 ? 16 : (bz <= 32 ? 32 : 64);
+        const bool staged = any && S > 0 && bx * by * S <= kBoxCap;
+        if (staged) {
+            // box rows, 256 / lpr rows per pass; 16 passes' loads are issued
+            // before any LDS store so their latencies overlap
+            const int rpi = 256 / lpr;
+            const int zl = tid & (lpr - 1), rl = tid / lpr;
+            const int nrows = bx * by;
+            const int gz = lo[2] + zl;
+            const float inv_by = 1.0f / (float)by;
+            const int npass = (nrows + rpi - 1) / rpi;
+            for (int p0 = 0; p0 < npass; p0 += 16) {
+                uint32_t v[16];
+                int dst[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int r = (p0 + u) * rpi + rl;
+                    const bool in = (p0 + u) < npass && r < nrows && zl < bz;
+                    // r / by for r < 2^16: the float estimate is within 0.5/by of an integer
+                    const int x = (int)(((float)r + 0.5f) * inv_by), y = r - x * by;
+                    const int gidx = in ? ((lo[0] + x) * np + (lo[1] + y)) * np + gz : 0;
+                    v[u] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, gidx * 4, 0, 0);
+                    dst[u] = in ? r * S + zl : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (dst[u] >= 0) s_box[dst[u]] = __uint_as_float(v[u]);
+            }
+        }
+        __syncthreads();
+        // ---- taps: each wave its 64 points, PPW per step
+#pragma unroll 1
+        for (int it = 0; it < G; ++it) {
+            const int pl = wave * 64 + it * PPW + g;
+            const int4 pi = s_ijk[pl];
+            const float4 tt = s_t[pl];
+            float wa_c = 0.f, wb_c = 0.f, wc_c = 0.f;
+            if (c < W) {
+                wa_c = es_phi((float)c - tt.x, gv.beta_l2e, gv.c2);
+                wb_c = es_phi((float)c - tt.y, gv.beta_l2e, gv.c2);
+                wc_c = es_phi((float)c - tt.z, gv.beta_l2e, gv.c2);
+            }
+            s_w[wave][g][0][c] = wa_c;
+            s_w[wave][g][1][c] = wb_c;
+            wave_sync();
+            float wa[W], wb[W];
+#pragma unroll
+            for (int a = 0; a < W; ++a) {
+                wa[a] = s_w[wave][g][0][a];
+                wb[a] = s_w[wave][g][1][a];
+            }
+            const bool valid = pi.w >= 0;
+            float acc = 0.f;
+            if (staged) {
+                int lb = 0;
+                if (valid) lb = ((pi.x - lo[0]) * by + (pi.y - lo[1])) * S + (pi.z - lo[2]) + c;
+                switch (S) {
+#define MTIP_TAPS_CASE(SS) case SS: acc = taps_lds<W, G, SS>(s_box, lb, by, wa, wb); break;
+                    MTIP_STRIDE_CASES(MTIP_TAPS_CASE)
+#undef MTIP_TAPS_CASE
+                default: break;
+                }
+            } else {
+                const int voff = valid ? ((pi.x * np + pi.y) * np + pi.z + c) * 4 : 0;
+                acc = taps_direct<W>(rsrc, np, voff, wa, wb);
+            }
+            float r = acc * wc_c;
+#pragma unroll
+            for (int m = G / 2; m >= 1; m >>= 1) r += __shfl_xor(r, m, 64);
+            if (c == 0 && pi.w != -1) {
+                float v = r;
+                int j = pi.w;
+                if (pi.w == -2) {
+                    const int64_t k = tb + pl;
+                    j = order ? order[k] : (int)k;
+                    v = __builtin_nanf("");
+                    ++nbad;
+                }
+                orow[j] = v;
+                if (brow) brow[j] = f32_to_bf16_rne(v);
+                sum += (double)v;
+                sumsq += (double)v * (double)v;
+            }
+            wave_sync();
+        }
+        __syncthreads();
     }
     if (nbad && bad) atomicAdd(bad, nbad);
     if (row_sum || row_sumsq) {
@@ -367,19 +706,22 @@ __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ qu
 
 template <int W>
 void launch_slices(const GridView &gv, const double *quats, int64_t N_o, const double *q,
-                   int64_t N_p, double gscale, float *out, int64_t ld, uint16_t *obf,
-                   double *rs, double *rq, int *bad, hipStream_t st)
+                   int64_t N_p, const int32_t *order, int64_t n_order, double gscale, float *out,
+                   int64_t ld, uint16_t *obf, double *rs, double *rq, int *bad, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_slices<W>, dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q, N_p,
-                       gscale, out, ld, obf, rs, rq, bad);
+    constexpr int G = W <= 4 ? 4 : (W <= 8 ? 8 : 16);
+    hipLaunchKernelGGL((k_slices_lds<W, G>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q,
+                       N_p, order, n_order, gscale, out, ld, obf, rs, rq, bad);
 }
 
 template <int W>
 void launch_execute(const GridView &gv, const double *x, const double *y, const double *z,
                     int64_t N, float *out, int *bad, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_execute<W>, dim3((unsigned)mtip::ceil_div(N, 256)), dim3(256), 0, st,
-                       gv, x, y, z, N, out, bad);
+    constexpr int G = W <= 4 ? 4 : (W <= 8 ? 8 : 16);
+    hipLaunchKernelGGL((k_interp<W, G, GEOM_POINTS>), dim3((unsigned)mtip::ceil_div(N, 256)),
+                       dim3(256), 0, st, gv, nullptr, x, y, z, N, (double)gv.n / (2.0 * M_PI),
+                       out, (int64_t)0, nullptr, nullptr, nullptr, bad);
 }
 
 #define MTIP_W_DISPATCH(W, FN, ...)                                                          \
@@ -409,8 +751,10 @@ GridView view_of(const mtip_plan *p)
     gv.np = p->np;
     gv.off = p->off;
     gv.n = p->n;
-    gv.beta = (float)p->beta;
+    gv.beta_l2e = (float)(p->beta * 1.4426950408889634);
     gv.c2 = 4.0f / (float)(p->w * p->w);
+    gv.bytes = (uint32_t)((size_t)p->np * p->np * p->np * sizeof(float));
+    gv.inv_n = 1.0 / (double)p->n;
     return gv;
 }
 
@@ -433,7 +777,7 @@ extern "C" int mtip_plan_create(int M, double eps, int device, mtip_plan **out)
 {
     MTIP_REQUIRE(out, MTIP_ERR_ARG, "mtip_plan_create: null output");
     *out = nullptr;
-    MTIP_REQUIRE(M >= 2 && M <= 1024, MTIP_ERR_ARG, "mtip_plan_create: M must be in [2, 1024]");
+    MTIP_REQUIRE(M >= 2 && M <= 320, MTIP_ERR_ARG, "mtip_plan_create: M must be in [2, 320]");
     MTIP_REQUIRE(eps > 0.0 && eps < 1.0, MTIP_ERR_ARG, "mtip_plan_create: eps must be in (0, 1)");
     DeviceGuard dg(device);
     MTIP_REQUIRE(dg.ok, MTIP_ERR_HIP, "mtip_plan_create: cannot select device");
@@ -451,7 +795,8 @@ extern "C" int mtip_plan_create(int M, double eps, int device, mtip_plan **out)
     p->beta = bon * w;
     int n0 = 2 * M > 2 * w ? 2 * M : 2 * w;
     p->n = next235even(n0);
-    p->pad = w + 1;
+    int G = w <= 4 ? 4 : (w <= 8 ? 8 : 16);
+    p->pad = G + 1;     // window start clamp + G-lane row reads stay inside
     p->np = p->n + 2 * p->pad;
     p->off = p->n / 2 + p->pad;
 
@@ -563,7 +908,8 @@ extern "C" int mtip_plan_execute(mtip_plan *p, const double *x, const double *y,
 }
 
 extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, const double *q,
-                                int64_t N_p, double scale, float *out, int64_t ld_out,
+                                int64_t N_p, const int32_t *pixel_order, int64_t n_order,
+                                double scale, float *out, int64_t ld_out,
                                 uint16_t *out_bf16, double *row_sum, double *row_sumsq,
                                 int *bad_count, void *stream)
 {
@@ -575,12 +921,15 @@ extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, 
     MTIP_REQUIRE(std::isfinite(scale), MTIP_ERR_ARG, "mtip_plan_slices: scale not finite");
     if (N_o == 0 || N_p == 0) return MTIP_OK;
     MTIP_REQUIRE(quats && q && out, MTIP_ERR_ARG, "mtip_plan_slices: null pointer");
+    MTIP_REQUIRE(N_p <= 0x7fffffff, MTIP_ERR_ARG, "mtip_plan_slices: N_p too large");
+    MTIP_REQUIRE(!pixel_order || (n_order > 0 && n_order % 256 == 0), MTIP_ERR_ARG,
+                 "mtip_plan_slices: n_order must be a positive multiple of 256");
     DeviceGuard dg(p->device);
     GridView gv = view_of(p);
     // grid units per q unit: (pi / (E os)) * n / (2 pi)
     double gscale = scale * (double)p->n / (2.0 * M_PI);
-    MTIP_W_DISPATCH(p->w, launch_slices, gv, quats, N_o, q, N_p, gscale, out, ld_out, out_bf16,
-                    row_sum, row_sumsq, bad_count, (hipStream_t)stream);
+    MTIP_W_DISPATCH(p->w, launch_slices, gv, quats, N_o, q, N_p, pixel_order, n_order, gscale,
+                    out, ld_out, out_bf16, row_sum, row_sumsq, bad_count, (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }

@@ -91,6 +91,34 @@ def _span(timers, name):
     return timers.span(name) if timers is not None else _NoSpan()
 
 
+def tile_order(det_shape, tile=(16, 16)):
+    """Pixel visiting order in runs of 256: 16 x 16 blocks of each panel, -1 padded.
+
+    det_shape (..., rows, cols) with leading dims as panels; returns an int32
+    numpy array whose length is a multiple of 256, or None for a 1-D detector.
+    """
+    det_shape = tuple(int(d) for d in det_shape)
+    if len(det_shape) < 2:
+        return None
+    rows, cols = det_shape[-2:]
+    panels = int(np.prod(det_shape[:-2])) if len(det_shape) > 2 else 1
+    ty, tx = tile
+    idx = np.arange(panels * rows * cols, dtype=np.int64).reshape(panels, rows, cols)
+    runs = []
+    for p in range(panels):
+        for r in range(0, rows, ty):
+            for c in range(0, cols, tx):
+                blk = idx[p, r:r + ty, c:c + tx].ravel()
+                run = np.full(ty * tx, -1, dtype=np.int64)
+                run[:blk.size] = blk
+                runs.append(run)
+    out = np.concatenate(runs)
+    pad = (-out.size) % 256
+    if pad:
+        out = np.concatenate([out, np.full(pad, -1)])
+    return out.astype(np.int32)
+
+
 class NufftPlan:
     """Type-2 NUFFT plan of a cubic M^3 model (ES kernel, sigma = 2)."""
 
@@ -155,7 +183,7 @@ class NufftPlan:
         return out
 
     def slices(self, quats, q, scale, out=None, ld=None, want_bf16=False, bad=None, stream=None,
-               timers=None):
+               timers=None, order=None):
         """Slices at N_o orientations for pixel q-vectors q (3, N_p).
 
         quats: (N_o, 4) float64 device tensor; q: (3, N_p) float64 device tensor.
@@ -167,6 +195,10 @@ class NufftPlan:
         q = q.to(self.device, torch.float64).contiguous()
         N_o = quats.shape[0]
         N_p = q.shape[1]
+        if order is not None:
+            if isinstance(order, np.ndarray):
+                order = torch.as_tensor(order)
+            order = order.to(self.device, torch.int32).contiguous()
         if ld is None:
             ld = padded(N_p)
         if out is None:
@@ -177,7 +209,8 @@ class NufftPlan:
         if bad is None:
             bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         with _span(timers, "nufft"):
-            rc = self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, float(scale),
+            rc = self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, _ptr(order),
+                                           order.numel() if order is not None else 0, float(scale),
                                            _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
                                            _stream(stream))
         _lib.check(rc, "mtip_plan_slices")
@@ -382,14 +415,14 @@ class SliceSet:
         self._center = None
 
     @classmethod
-    def compute(cls, plan, quats, q, scale, precision="fp32", stream=None, timers=None):
+    def compute(cls, plan, quats, q, scale, precision="fp32", stream=None, timers=None, order=None):
         """NUFFT slices of the plan's model at quaternions `quats` (N_o, 4)."""
         dev = plan.device
         quats = torch.as_tensor(np.ascontiguousarray(quats, np.float64)).to(dev) \
             if isinstance(quats, np.ndarray) else quats.to(dev, torch.float64)
         q = torch.as_tensor(np.ascontiguousarray(q, np.float64)).to(dev) \
             if isinstance(q, np.ndarray) else q.to(dev, torch.float64)
-        r = plan.slices(quats, q, scale, stream=stream, timers=timers)
+        r = plan.slices(quats, q, scale, stream=stream, timers=timers, order=order)
         return cls(r["s32"], r["N_p"], r["row_sum"], r["row_sumsq"], r["bf16"], quats, r["bad"])
 
     @classmethod

@@ -21,25 +21,29 @@ def torch_dev():
     return torch.device("cuda", 0)
 
 
-def _gpu_slices(ac, quats, ppr, E, eps=1e-6, os_=1):
+def _gpu_slices(ac, quats, ppr, E, eps=1e-6, os_=1, tiled=True):
     import torch
     from spinifel_amd import engine
+    order = engine.tile_order(ppr.shape[1:]) if tiled else None
     plan = engine.NufftPlan(ac.shape[0], eps, 0)
     plan.set_model(ac)
     q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1)))
-    r = plan.slices(torch.as_tensor(quats), q, np.pi / (E * os_))
+    r = plan.slices(torch.as_tensor(quats), q, np.pi / (E * os_), order=order)
     assert int(r["bad"].item()) == 0
     n_p = q.shape[1]
     return r["s32"][:, :n_p].double().cpu().numpy(), r
 
 
-def test_slices_m41_vs_nudft(torch_dev):
-    print("case: M=41 1x32x32, 64 orientations", flush=True)
+@pytest.mark.parametrize("tiled", [True, False])
+def test_slices_m41_vs_nudft(torch_dev, tiled):
+    """8 x 8 pixel tiles (LDS-staged boxes) and index-order runs of 64 pixels
+    (boxes too big for LDS on this detector: direct-load fallback)."""
+    print(f"case: M=41 1x32x32, 64 orientations, tiled={tiled}", flush=True)
     ac = synth.make_model(41, 300, 0)
     ppr, pdr = synth.detector(1, 32, 32)
     E = pdr.max()
     quats = synth.random_quaternions(64, np.random.default_rng(5))
-    s, r = _gpu_slices(ac, quats, ppr, E)
+    s, r = _gpu_slices(ac, quats, ppr, E, tiled=tiled)
     H, K, L = normalized_points(quats, ppr, E)
     rng = np.random.default_rng(0)
     sub = rng.choice(H.size, 3000, replace=False)
