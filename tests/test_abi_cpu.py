@@ -1,0 +1,75 @@
+"""The C-ABI library loads, exports every declared symbol and rejects bad
+arguments before touching a GPU; the product path refuses to run without
+one (no CPU fallback)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from spinifel_amd import _lib
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    names = _lib.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared entry point has a ctypes signature in the binding
+    assert set(names) == set(_lib._SIGS), set(names) ^ set(_lib._SIGS)
+
+
+def test_abi_version_and_workspace_size():
+    lib = _lib.load()
+    assert lib.mtip_abi_version() == 1
+    assert lib.mtip_match_workspace_size(10000, 10000, 8, 1) == 10000 * 79 * 8 * 8
+    assert lib.mtip_match_workspace_size(1, 1, 1, 0) == 8
+    assert lib.mtip_match_workspace_size(0, 5, 1, 0) == 0
+
+
+def test_argument_errors_are_reported_without_a_gpu():
+    lib = _lib.load()
+    nul = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)    # never dereferenced: argument checks fail first
+    rc = lib.mtip_match_topk(7, fake, 16, nul, fake, 4, fake, 16, nul, 4, 16, fake, 1, fake, 0,
+                             fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG
+    assert "precision" in _lib.last_error()
+    rc = lib.mtip_match_topk(0, fake, 16, nul, fake, 4, fake, 16, nul, 4, 16, fake, 3, fake, 0,
+                             fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "K must be" in _lib.last_error()
+    rc = lib.mtip_match_topk(0, fake, 32, nul, fake, 4, fake, 32, nul, 4, 20, fake, 1, fake, 0,
+                             fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "multiple of 16" in _lib.last_error()
+    rc = lib.mtip_match_topk(0, fake, 16, nul, fake, 4, fake, 16, nul, 4, 16, fake, 1, fake, 0,
+                             fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_NOMEM and "workspace" in _lib.last_error()
+    rc = lib.mtip_match_topk(2, fake, 16, nul, fake, 4, fake, 16, nul, 4, 16, fake, 1, fake, 4096,
+                             fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "FP8" in _lib.last_error()
+    rc = lib.mtip_heap_sort(nul, 4, 4, 9, nul, nul)
+    assert rc == _lib.MTIP_ERR_ARG
+    rc = lib.mtip_plan_create(1, 1e-6, 0, ctypes.byref(ctypes.c_void_p()))
+    assert rc == _lib.MTIP_ERR_ARG and "M must be" in _lib.last_error()
+    rc = lib.mtip_residual_rows(nul, 4, 8, 4, nul, 0, nul, 0, nul, 8, nul, nul, nul)
+    assert rc == _lib.MTIP_ERR_ARG
+    with pytest.raises(ValueError):
+        _lib.check(_lib.MTIP_ERR_ARG, "x")
+    with pytest.raises(AssertionError):
+        _lib.check(_lib.MTIP_ERR_DOMAIN, "x")
+    with pytest.raises(MemoryError):
+        _lib.check(_lib.MTIP_ERR_NOMEM, "x")
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from spinifel_amd import engine, orientation_matching, settings as st
+    with pytest.raises(RuntimeError, match="GPU"):
+        engine.NufftPlan(21, 1e-6, 0)
+    s = st.SpinifelSettings(N_orientations=4, N_batch_size=4, reduced_det_shape=(1, 4, 4))
+    with pytest.raises(RuntimeError, match="GPU"):
+        orientation_matching.slicing_and_match(
+            np.ones((5, 5, 5), np.float32), np.ones((2, 1, 4, 4), np.float32),
+            np.ones((3, 1, 4, 4)), np.ones((1, 4, 4)), np.eye(4), settings=s)

@@ -1,0 +1,59 @@
+"""Host-side pieces: synthetic workload generators, tile order, settings."""
+import numpy as np
+import pytest
+
+from spinifel_amd import engine, settings as st, synth
+
+
+def test_model_is_real_centrosymmetric_autocorrelation():
+    ac = synth.make_model(21, 30, 0)
+    assert ac.dtype == np.float32 and ac.shape == (21, 21, 21)
+    np.testing.assert_array_equal(ac, ac[::-1, ::-1, ::-1])
+    rho = synth.density(21, 30, synth.rng_for(0, synth.STREAM_ATOMS))
+    # the centre of an autocorrelation is sum(rho^2), its maximum
+    assert ac[10, 10, 10] == pytest.approx((rho ** 2).sum(), rel=1e-5)
+    assert ac.max() == ac[10, 10, 10]
+    np.testing.assert_array_equal(ac, synth.make_model(21, 30, 0))
+
+
+def test_detector_geometry():
+    ppr, pdr = synth.detector(4, 32, 128)
+    assert ppr.shape == (3, 4, 32, 128) and pdr.shape == (4, 32, 128)
+    q = np.linalg.norm(ppr, axis=0)
+    np.testing.assert_allclose(pdr, q / 0.9)
+    # Ewald sphere: |q + z_hat / lambda| = 1 / lambda
+    np.testing.assert_allclose(np.linalg.norm(ppr + np.array([0, 0, 1.0])[:, None, None, None], axis=0),
+                               1.0, rtol=1e-12)
+
+
+def test_quaternions_and_images():
+    q = synth.random_quaternions(1000, np.random.default_rng(0))
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0)
+    assert (q[:, 0] >= 0).all()
+    s = np.abs(np.random.default_rng(1).standard_normal((400, 64)))
+    im = synth.poisson_images(s, 2.0, np.random.default_rng(2))
+    assert im.dtype == np.float32
+    assert abs(im.mean() - 2.0) < 0.1
+    assert (im == np.round(im)).all()
+
+
+@pytest.mark.parametrize("shape", [(4, 32, 128), (1, 32, 32), (1, 20, 13), (3, 7, 40)])
+def test_tile_order_is_a_permutation(shape):
+    o = engine.tile_order(shape)
+    assert o.size % 256 == 0
+    valid = o[o >= 0]
+    np.testing.assert_array_equal(np.sort(valid), np.arange(int(np.prod(shape))))
+    # each run of 256 stays inside one 16 x 16 block of one panel
+    rows, cols = shape[-2:]
+    for run in o.reshape(-1, 256):
+        v = run[run >= 0]
+        if v.size:
+            p, r, c = np.unravel_index(v, (int(np.prod(shape[:-2])), rows, cols))
+            assert len(set(p)) == 1 and r.max() - r.min() < 16 and c.max() - c.min() < 16
+    assert engine.tile_order((100,)) is None
+
+
+def test_settings_shim():
+    s = st.SpinifelSettings()
+    assert s.Mquat == 20 and s.M == 81
+    assert st.SpinifelSettings(oversampling=2).M == 161
