@@ -186,6 +186,8 @@ __global__ __launch_bounds__(256) void k_dft_pass(const float *__restrict__ ac,
 // the group shares them through LDS; the group sums its lanes' partial
 // results with xor-shuffles.
 
+typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
+
 struct GridView {
     const float *__restrict__ g;
     int np, off, n;
@@ -472,6 +474,25 @@ __device__ __forceinline__ float taps_direct(__amdgpu_buffer_rsrc_t rsrc, int np
     return acc;
 }
 
+// Sum over aligned groups of G lanes; the result is valid in lane 0 of each
+// group.  DPP moves (VALU) instead of LDS permutes: quad_perm for xor 1 / 2,
+// row_shl 4 / 8 for the other quads of the 8- / 16-lane group.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v)
+{
+    if (G >= 2) v += dpp_f<0xB1>(v);     // quad_perm [1,0,3,2]: lane ^ 1
+    if (G >= 4) v += dpp_f<0x4E>(v);     // quad_perm [2,3,0,1]: lane ^ 2
+    if (G >= 8) v += dpp_f<0x104>(v);    // row_shl:4: lane + 4 within the row of 16
+    if (G >= 16) v += dpp_f<0x108>(v);   // row_shl:8: lane + 8 within the row of 16
+    return v;
+}
+
 __device__ __forceinline__ int wave_min_i(int v)
 {
 #pragma unroll
@@ -547,22 +568,42 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
     int nbad = 0;
     const int64_t n_slots = order ? n_order : mtip::ceil_div(N_p, kTile) * kTile;
 
+    // pixel index and q-vector of this thread's point, prefetched one tile ahead
+    // (and the pixel index two tiles ahead) so their latency hides behind taps
+    auto pixel_at = [&](int64_t k) -> int {
+        if (order) return (k < n_order) ? order[k] : -1;
+        return (k < N_p) ? (int)k : -1;
+    };
+    int pix_n = pixel_at(tid);
+    double qx_n = 0.0, qy_n = 0.0, qz_n = 0.0;
+    {
+        const int pp = pix_n >= 0 ? pix_n : 0;
+        qx_n = q[pp];
+        qy_n = q[N_p + pp];
+        qz_n = q[2 * N_p + pp];
+    }
+    int pix_nn = pixel_at(kTile + tid);
+
     for (int64_t tb = 0; tb < n_slots; tb += kTile) {
         // ---- geometry of this thread's point
-        int pix = -1, ok = 0;
+        const int pix = pix_n;
+        const double x = qx_n, y = qy_n, z = qz_n;
+        pix_n = pix_nn;
+        {
+            const int pp = pix_n >= 0 ? pix_n : 0;
+            qx_n = q[pp];
+            qy_n = q[N_p + pp];
+            qz_n = q[2 * N_p + pp];
+        }
+        pix_nn = pixel_at(tb + 2 * kTile + tid);
+        int ok = 0;
         int ijk[3] = {0, 0, 0};
         float t[3] = {0.f, 0.f, 0.f};
-        {
-            const int64_t k = tb + tid;
-            if (order) pix = (k < n_order) ? order[k] : -1;
-            else pix = (k < N_p) ? (int)k : -1;
-            if (pix >= 0) {
-                double x = q[pix], y = q[N_p + pix], z = q[2 * N_p + pix];
-                double X0 = (ri[0][0] * x + ri[0][1] * y + ri[0][2] * z) * gscale;
-                double X1 = (ri[1][0] * x + ri[1][1] * y + ri[1][2] * z) * gscale;
-                double X2 = (ri[2][0] * x + ri[2][1] * y + ri[2][2] * z) * gscale;
-                ok = point_corner<W>(gv, X0, X1, X2, ijk, t) ? 1 : 0;
-            }
+        if (pix >= 0) {
+            double X0 = (ri[0][0] * x + ri[0][1] * y + ri[0][2] * z) * gscale;
+            double X1 = (ri[1][0] * x + ri[1][1] * y + ri[1][2] * z) * gscale;
+            double X2 = (ri[2][0] * x + ri[2][1] * y + ri[2][2] * z) * gscale;
+            ok = point_corner<W>(gv, X0, X1, X2, ijk, t) ? 1 : 0;
         }
         s_ijk[tid] = make_int4(ijk[0], ijk[1], ijk[2], ok ? pix : (pix >= 0 ? -2 : -1));
         s_t[tid] = make_float4(t[0], t[1], t[2], 0.f);
@@ -591,30 +632,39 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
         const int lpr = bz <= 16 ? 16 : (bz <= 32 ? 32 : 64);
         const bool staged = any && S > 0 && bx * by * S <= kBoxCap;
         if (staged) {
-            // box rows, 256 / lpr rows per pass; 16 passes' loads are issued
-            // before any LDS store so their latencies overlap
-            const int rpi = 256 / lpr;
-            const int zl = tid & (lpr - 1), rl = tid / lpr;
+            // box rows in 16-byte chunks: l4 threads per row (a power of two >=
+            // ceil(bz / 4)), 256 / l4 rows per pass, 8 passes' loads in flight
+            // before their LDS stores
+            const int q4 = (bz + 3) >> 2;
+            const int l4 = q4 <= 4 ? 4 : (q4 <= 8 ? 8 : 16);
+            const int rpi = 256 / l4;
+            const int zq = tid & (l4 - 1), rl = tid / l4;
+            const int z0 = 4 * zq;
             const int nrows = bx * by;
-            const int gz = lo[2] + zl;
+            const int gz = lo[2] + z0;
             const float inv_by = 1.0f / (float)by;
             const int npass = (nrows + rpi - 1) / rpi;
-            for (int p0 = 0; p0 < npass; p0 += 16) {
-                uint32_t v[16];
-                int dst[16];
+            for (int p0 = 0; p0 < npass; p0 += 8) {
+                uint4v v[8];
+                int dst[8];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
+                for (int u = 0; u < 8; ++u) {
                     const int r = (p0 + u) * rpi + rl;
-                    const bool in = (p0 + u) < npass && r < nrows && zl < bz;
+                    const bool in = (p0 + u) < npass && r < nrows && zq < q4;
                     // r / by for r < 2^16: the float estimate is within 0.5/by of an integer
                     const int x = (int)(((float)r + 0.5f) * inv_by), y = r - x * by;
                     const int gidx = in ? ((lo[0] + x) * np + (lo[1] + y)) * np + gz : 0;
-                    v[u] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, gidx * 4, 0, 0);
-                    dst[u] = in ? r * S + zl : -1;
+                    v[u] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, gidx * 4, 0, 0);
+                    dst[u] = in ? r * S + z0 : -1;
                 }
 #pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    if (dst[u] >= 0) s_box[dst[u]] = __uint_as_float(v[u]);
+                for (int u = 0; u < 8; ++u) {
+                    if (dst[u] >= 0) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (z0 + e < bz) s_box[dst[u] + e] = __uint_as_float(v[u][e]);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -654,9 +704,7 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
                 const int voff = valid ? ((pi.x * np + pi.y) * np + pi.z + c) * 4 : 0;
                 acc = taps_direct<W>(rsrc, np, voff, wa, wb);
             }
-            float r = acc * wc_c;
-#pragma unroll
-            for (int m = G / 2; m >= 1; m >>= 1) r += __shfl_xor(r, m, 64);
+            const float r = group_sum<G>(acc * wc_c);
             if (c == 0 && pi.w != -1) {
                 float v = r;
                 int j = pi.w;
