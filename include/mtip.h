@@ -170,9 +170,14 @@ int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_i
                        const double *mult, int invert_mult, const float *center, int format,
                        void *out, int64_t ld_out, float *scales, double *sumsq, void *stream);
 
-/* Column means (FP64 accumulation) of a float32 (rows, cols) matrix -> float32 (cols). */
-int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, float *mean,
-                  void *stream);
+/*
+ * Column means (FP64 accumulation, fixed summation order) of a float32
+ * (rows, cols) matrix -> float32 (cols).  work: device scratch of
+ * mtip_col_mean_workspace_size(cols) bytes.
+ */
+size_t mtip_col_mean_workspace_size(int64_t cols);
+int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, double *work,
+                  size_t work_bytes, float *mean, void *stream);
 
 /* ------------------------------------------------------------- matching */
 
@@ -206,17 +211,18 @@ int mtip_match_topk(int precision, const void *slices, int64_t ld_s, const float
 
 /*
  * The two halves of mtip_match_topk, for callers that time or reuse them:
- * mtip_match_partial runs the MFMA GEMM and writes per-(image, 128-row
- * orientation tile) top-K packed keys to the workspace; mtip_topk_merge
- * reduces them to the per-image top-K.  Same arguments and rules as
- * mtip_match_topk.
+ * mtip_match_partial runs the MFMA GEMM and writes per-(image, orientation
+ * tile) top-K packed keys to the workspace (128- or 256-row tiles, chosen
+ * from precision and sizes); mtip_topk_merge reduces them to the per-image
+ * top-K and must get the same precision and sizes.  Same arguments and
+ * rules as mtip_match_topk.
  */
 int mtip_match_partial(int precision, const void *slices, int64_t ld_s, const float *slice_scale,
                        const double *slice_sumsq, int64_t N_o, const void *images, int64_t ld_d,
                        const float *image_scale, int64_t N_i, int64_t N_p, const double *alpha,
                        int K, void *workspace, size_t workspace_bytes, void *stream);
-int mtip_topk_merge(const void *workspace, int64_t N_o, int64_t N_i, int K, int32_t *top_idx,
-                    float *top_val, void *stream);
+int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i, int K,
+                    int32_t *top_idx, float *top_val, void *stream);
 
 /*
  * Rescore K candidates per image on FP32 slices/images:

@@ -47,13 +47,14 @@ _SIGS = {
     "mtip_quantize_rows": (_i32, [_vp, _i64, _i64, _i64, _i32, _vp, _i64, _vp, _vp]),
     "mtip_residual_rows": (_i32, [_vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp, _i64, _vp,
                                   _vp, _vp]),
-    "mtip_col_mean": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp]),
+    "mtip_col_mean_workspace_size": (_c.c_size_t, [_i64]),
+    "mtip_col_mean": (_i32, [_vp, _i64, _i64, _i64, _vp, _c.c_size_t, _vp, _vp]),
     "mtip_match_workspace_size": (_c.c_size_t, [_i64, _i64, _i32, _i32]),
     "mtip_match_topk": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                                _vp, _i32, _vp, _c.c_size_t, _vp, _vp, _vp]),
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                                   _vp, _i32, _vp, _c.c_size_t, _vp]),
-    "mtip_topk_merge": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp]),
+    "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
     "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _vp, _i32, _vp, _vp,
                             _vp, _vp, _vp]),
     "mtip_euclidean_distance": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),

@@ -273,6 +273,227 @@ __global__ __launch_bounds__(256) void k_gemm(const uint8_t *__restrict__ A, int
     }
 }
 
+// ---------------------------------------------------------------- big tiles
+//
+// 256 (orientations) x 256 (images) per 512-thread workgroup for large
+// problems: 8 waves in a 2 (M) x 4 (N) grid of 128 x 64 sub-tiles, each a
+// 4 x 2 set of 32 x 32 MFMA blocks.  Two LDS stages (double buffer, one
+// barrier per K step) with the next-but-one stage prefetched into registers.
+// Twice the flops per staged byte of the 128 x 128 kernel, which is bound by
+// L2 -> LDS traffic at bf16 / e4m3 rates.  Tile order: workgroup ids are
+// dealt to the 8 XCDs round robin, so each XCD gets a contiguous range of
+// tiles, walked 8 tile-rows at a time, and the workgroups running together
+// on one XCD share A and B panels in its L2.
+constexpr int BM2 = 256, BN2 = 256;
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ void tile_of(int bid, int nblocks, int nta, int ntb, int &ta, int &tb)
+{
+    const int per = (nblocks + kXcds - 1) / kXcds;
+    const int xcd = bid % kXcds, local = bid / kXcds;
+    int t = xcd * per + local;
+    if (t >= nblocks) t = bid;        // uneven split: fall back to the launch order
+    constexpr int GM = 8;
+    const int in_group = GM * ntb;
+    const int group = t / in_group;
+    const int first = group * GM;
+    const int gm = min(nta - first, GM);
+    const int r = t % in_group;
+    ta = first + r % gm;
+    tb = r / gm;
+}
+
+template <int PREC, int K>
+__global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A, int64_t lda,
+                                                  int64_t NA, const uint8_t *__restrict__ B,
+                                                  int64_t ldb, int64_t NB, int64_t kbytes,
+                                                  const float *__restrict__ scaleA,
+                                                  const float *__restrict__ scaleB,
+                                                  const double *__restrict__ sumsqA,
+                                                  const double *__restrict__ alpha, int nta,
+                                                  int ntb, uint64_t *__restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t smem[2 * (BM2 + BN2) * LDS_STRIDE];
+    __shared__ float s_rowc[BM2];
+    __shared__ float s_sa[BM2], s_sb[BN2];
+    constexpr int STAGE = (BM2 + BN2) * LDS_STRIDE;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int half = lane >> 5, l32 = lane & 31;
+    int ta, tb;
+    tile_of(blockIdx.x, nta * ntb, nta, ntb, ta, tb);
+    const int64_t a0 = (int64_t)ta * BM2, b0 = (int64_t)tb * BN2;
+
+    if (tid < BM2) {
+        const int64_t r = a0 + tid;
+        float sc = 1.0f;
+        if (PREC == MTIP_FP8 && r < NA) sc = scaleA[r];
+        s_sa[tid] = sc;
+        const double al = *alpha;
+        s_rowc[tid] = r < NA ? (float)(al * al * sumsqA[r]) : 0.0f;
+    } else {
+        const int64_t r = b0 + (tid - BM2);
+        float sc = 1.0f;
+        if (PREC == MTIP_FP8 && r < NB) sc = scaleB[r];
+        s_sb[tid - BM2] = sc;
+    }
+
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    // staging: 2048 16-byte chunks per operand tile, 4 per thread per operand
+    uint4 ra[4], rb[4];
+    auto load_stage = [&](int64_t kb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = tid + 512 * i;
+            const int r = c >> 3, col = (c & 7) * 16;
+            const int64_t gk = kb + col;
+            const int64_t ga = a0 + r, gb = b0 + r;
+            ra[i] = (ga < NA && gk < kbytes) ? load16(A + ga * lda + gk) : make_uint4(0, 0, 0, 0);
+            rb[i] = (gb < NB && gk < kbytes) ? load16(B + gb * ldb + gk) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store_stage = [&](int buf) {
+        uint8_t *sA = smem + buf * STAGE;
+        uint8_t *sB = sA + BM2 * LDS_STRIDE;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int c = tid + 512 * i;
+            const int r = c >> 3, col = (c & 7) * 16;
+            *(uint4 *)(sA + r * LDS_STRIDE + col) = ra[i];
+            *(uint4 *)(sB + r * LDS_STRIDE + col) = rb[i];
+        }
+    };
+
+    const int nstage = (int)((kbytes + BKB - 1) / BKB);
+    load_stage(0);
+    store_stage(0);
+    if (nstage > 1) load_stage(BKB);
+    __syncthreads();
+    for (int st = 0; st < nstage; ++st) {
+        const uint8_t *sA = smem + (st & 1) * STAGE;
+        const uint8_t *sB = sA + BM2 * LDS_STRIDE;
+        const uint8_t *ar[4], *br[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ar[i] = sA + (wm * 128 + i * 32 + l32) * LDS_STRIDE;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) br[j] = sB + (wn * 64 + j * 32 + l32) * LDS_STRIDE;
+        if constexpr (PREC == MTIP_FP32) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int off = g * 32 + half * 16;
+                floatx4 av[4], bv[2];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) av[i] = *(const floatx4 *)(ar[i] + off);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[j] = *(const floatx4 *)(br[j] + off);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][t], bv[j][t], acc[i][j], 0, 0, 0);
+            }
+        } else if constexpr (PREC == MTIP_BF16) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int off = s4 * 32 + half * 16;
+                bf16x8 av[4], bv[2];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) av[i] = *(const bf16x8 *)(ar[i] + off);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[j] = *(const bf16x8 *)(br[j] + off);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int off = s2 * 64 + half * 32;
+                intx8 bv[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[j] = *(const intx8 *)(br[j] + off);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const intx8 av = *(const intx8 *)(ar[i] + off);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv[j], acc[i][j], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        }
+        if (st + 1 < nstage) {
+            store_stage((st + 1) & 1);
+            if (st + 2 < nstage) load_stage((int64_t)(st + 2) * BKB);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: per image column the K smallest v over the tile's 256 rows
+    const float two_a = 2.0f * (float)(*alpha);
+    uint64_t top[2][K];
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) top[bn][r] = KEY_NONE;
+        const int j = wn * 64 + bn * 32 + l32;
+        const float sb = s_sb[j];
+#pragma unroll
+        for (int bm = 0; bm < 4; ++bm) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int m = wm * 128 + bm * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+                const int64_t go = a0 + m;
+                if (go < NA) {
+                    float c = acc[bm][bn][v];
+                    if (PREC == MTIP_FP8) c *= s_sa[m] * sb;
+                    const float val = fmaf(-two_a, c, s_rowc[m]);
+                    mtip::topk_insert<K>(top[bn], mtip::make_key(val, (uint32_t)go));
+                }
+            }
+        }
+        uint64_t other[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[bn][r], 32);
+        mtip::topk_merge<K>(top[bn], other);
+    }
+    uint64_t *s_top = (uint64_t *)smem;    // [4 wn][64 cols][K] (after the final barrier)
+    if (wm == 1 && half == 0) {
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+            for (int r = 0; r < K; ++r) s_top[(wn * 64 + bn * 32 + l32) * K + r] = top[bn][r];
+    }
+    __syncthreads();
+    if (wm == 0 && half == 0) {
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) {
+            const int j = wn * 64 + bn * 32 + l32;
+            uint64_t other[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r) other[r] = s_top[j * K + r];
+            mtip::topk_merge<K>(top[bn], other);
+            const int64_t gi = b0 + j;
+            if (gi < NB) {
+                uint64_t *dst = part + (gi * nta + ta) * K;
+#pragma unroll
+                for (int r = 0; r < K; ++r) dst[r] = top[bn][r];
+            }
+        }
+    }
+}
+
 // One wave per row: K smallest keys of `n_keys` packed keys (KEYS = true)
 // or of `n_keys` floats (KEYS = false, key = (value, column)).
 template <int K, bool KEYS>
@@ -551,32 +772,70 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ in, 
     }
 }
 
-// Column means (FP64 accumulation) of a float32 (rows, cols) matrix: block of
-// 256 = 64 columns x 4 row phases, partial sums combined in LDS.
-__global__ __launch_bounds__(256) void k_col_mean(const float *__restrict__ in, int64_t rows,
-                                                  int64_t cols, int64_t ld, float *__restrict__ mean)
+// Column sums (FP64) of a float32 (rows, cols) matrix over kColGroups row
+// groups: block of 256 = 64 columns x 4 row phases, grid (column blocks, row
+// groups); partial sums land in work[group][col] and are added in a fixed
+// order by k_col_finish, so the result does not depend on scheduling.
+constexpr int kColGroups = 16;
+
+__global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ in, int64_t rows,
+                                                     int64_t cols, int64_t ld,
+                                                     double *__restrict__ work)
 {
     __shared__ double part[4][64];
     const int cx = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int64_t c = (int64_t)blockIdx.x * 64 + cx;
-    double s = 0.0;
-    if (c < cols)
-        for (int64_t r = ph; r < rows; r += 4) s += in[r * ld + c];
-    part[ph][cx] = s;
+    const int64_t per = (rows + kColGroups - 1) / kColGroups;
+    const int64_t r0 = (int64_t)blockIdx.y * per, r1 = min(rows, r0 + per);
+    double s0 = 0.0, s1 = 0.0;
+    if (c < cols) {
+        int64_t r = r0 + ph;
+        for (; r + 4 < r1; r += 8) {
+            s0 += in[r * ld + c];
+            s1 += in[(r + 4) * ld + c];
+        }
+        for (; r < r1; r += 4) s0 += in[r * ld + c];
+    }
+    part[ph][cx] = s0 + s1;
     __syncthreads();
     if (ph == 0 && c < cols)
-        mean[c] = (float)(((part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx])) / (double)rows);
+        work[(int64_t)blockIdx.y * cols + c] = (part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx]);
+}
+
+__global__ __launch_bounds__(256) void k_col_finish(const double *__restrict__ work, int64_t rows,
+                                                    int64_t cols, float *__restrict__ mean)
+{
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    double t = 0.0;
+    for (int g = 0; g < kColGroups; ++g) t += work[(int64_t)g * cols + c];
+    mean[c] = (float)(t / (double)rows);
+}
+
+
+// Orientation rows per top-K partial tile: 256 (big kernel) for large
+// problems, else 128.  The partial buffer and the merge use the same value.
+int tile_rows(int precision, int64_t N_o, int64_t N_i)
+{
+    return (precision == MTIP_BF16 && N_o >= 1024 && N_i >= 1024) ? BM2 : BM;
 }
 
 template <int PREC, int K>
 void launch_gemm_topk(const void *S, int64_t lds_b, int64_t N_o, const void *D, int64_t ldd_b,
                       int64_t N_i, int64_t kbytes, const float *ssc, const float *dsc,
-                      const double *s2, const double *alpha, int nta, int ntb, uint64_t *part,
-                      hipStream_t st)
+                      const double *s2, const double *alpha, uint64_t *part, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_gemm<PREC, K, OUT_TOPK>), dim3((unsigned)(nta * ntb)), dim3(256), 0, st,
-                       (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i, kbytes, ssc,
-                       dsc, s2, alpha, nta, part, nullptr, (int64_t)0);
+    if (tile_rows(PREC, N_o, N_i) == BM2) {
+        const int nta = (int)mtip::ceil_div(N_o, BM2), ntb = (int)mtip::ceil_div(N_i, BN2);
+        hipLaunchKernelGGL((k_gemm_big<PREC, K>), dim3((unsigned)(nta * ntb)), dim3(512), 0, st,
+                           (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i, kbytes,
+                           ssc, dsc, s2, alpha, nta, ntb, part);
+    } else {
+        const int nta = (int)mtip::ceil_div(N_o, BM), ntb = (int)mtip::ceil_div(N_i, BN);
+        hipLaunchKernelGGL((k_gemm<PREC, K, OUT_TOPK>), dim3((unsigned)(nta * ntb)), dim3(256), 0,
+                           st, (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i,
+                           kbytes, ssc, dsc, s2, alpha, nta, part, nullptr, (int64_t)0);
+    }
 }
 
 template <int K>
@@ -647,25 +906,23 @@ extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_
     MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM,
                  "mtip_match_partial: workspace too small");
     const int eb = prec_bytes(precision);
-    const int nta = (int)mtip::ceil_div(N_o, BM);
-    const int ntb = (int)mtip::ceil_div(N_i, BN);
     hipStream_t st = (hipStream_t)stream;
     uint64_t *part = (uint64_t *)workspace;
     const int64_t kbytes = N_p * eb;
 #define MTIP_GEMM_CASE(P)                                                                        \
     switch (K) {                                                                                 \
     case 1: launch_gemm_topk<P, 1>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
-                                   part, st); break;                                             \
+                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+            break;                                                                               \
     case 2: launch_gemm_topk<P, 2>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
-                                   part, st); break;                                             \
+                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+            break;                                                                               \
     case 4: launch_gemm_topk<P, 4>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,       \
-                                   part, st); break;                                             \
+                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+            break;                                                                               \
     default: launch_gemm_topk<P, 8>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,    \
-                                    slice_scale, image_scale, slice_sumsq, alpha, nta, ntb,      \
-                                    part, st); break;                                            \
+                                    slice_scale, image_scale, slice_sumsq, alpha, part, st);     \
+             break;                                                                              \
     }
     if (precision == MTIP_FP32) { MTIP_GEMM_CASE(MTIP_FP32) }
     else if (precision == MTIP_BF16) { MTIP_GEMM_CASE(MTIP_BF16) }
@@ -675,8 +932,8 @@ extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_
     return MTIP_OK;
 }
 
-extern "C" int mtip_topk_merge(const void *workspace, int64_t N_o, int64_t N_i, int K,
-                               int32_t *top_idx, float *top_val, void *stream)
+extern "C" int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i,
+                               int K, int32_t *top_idx, float *top_val, void *stream)
 {
     MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_topk_merge: K must be 1, 2, 4 or 8");
     MTIP_REQUIRE(N_o > 0 && N_i >= 0, MTIP_ERR_ARG, "mtip_topk_merge: bad size");
@@ -684,7 +941,7 @@ extern "C" int mtip_topk_merge(const void *workspace, int64_t N_o, int64_t N_i, 
     MTIP_REQUIRE(workspace && top_idx, MTIP_ERR_ARG, "mtip_topk_merge: null pointer");
     hipStream_t st = (hipStream_t)stream;
     const uint64_t *part = (const uint64_t *)workspace;
-    const int64_t nkeys = mtip::ceil_div(N_o, BM) * K;
+    const int64_t nkeys = mtip::ceil_div(N_o, tile_rows(precision, N_o, N_i)) * K;
     switch (K) {
     case 1: launch_topk_rows<1>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
     case 2: launch_topk_rows<2>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
@@ -707,7 +964,7 @@ extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
                                 ld_d, image_scale, N_i, N_p, alpha, K, workspace, workspace_bytes,
                                 stream);
     if (rc != MTIP_OK || N_i == 0) return rc;
-    return mtip_topk_merge(workspace, N_o, N_i, K, top_idx, top_val, stream);
+    return mtip_topk_merge(precision, workspace, N_o, N_i, K, top_idx, top_val, stream);
 }
 
 extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
@@ -798,14 +1055,24 @@ extern "C" int mtip_quantize_rows(const float *in, int64_t rows, int64_t cols, i
                               scales, nullptr, stream);
 }
 
-extern "C" int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, float *mean,
-                             void *stream)
+extern "C" size_t mtip_col_mean_workspace_size(int64_t cols)
+{
+    return cols > 0 ? (size_t)cols * kColGroups * sizeof(double) : 0;
+}
+
+extern "C" int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, double *work,
+                             size_t work_bytes, float *mean, void *stream)
 {
     MTIP_REQUIRE(rows > 0 && cols >= 0 && ld >= cols, MTIP_ERR_ARG, "mtip_col_mean: bad shape");
     if (cols == 0) return MTIP_OK;
-    MTIP_REQUIRE(in && mean, MTIP_ERR_ARG, "mtip_col_mean: null pointer");
-    hipLaunchKernelGGL(k_col_mean, dim3((unsigned)mtip::ceil_div(cols, 64)), dim3(256), 0,
-                       (hipStream_t)stream, in, rows, cols, ld, mean);
+    MTIP_REQUIRE(in && mean && work, MTIP_ERR_ARG, "mtip_col_mean: null pointer");
+    MTIP_REQUIRE(work_bytes >= mtip_col_mean_workspace_size(cols), MTIP_ERR_NOMEM,
+                 "mtip_col_mean: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_col_partial, dim3((unsigned)mtip::ceil_div(cols, 64), kColGroups),
+                       dim3(256), 0, st, in, rows, cols, ld, work);
+    hipLaunchKernelGGL(k_col_finish, dim3((unsigned)mtip::ceil_div(cols, 256)), dim3(256), 0, st,
+                       work, rows, cols, mean);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }

@@ -293,8 +293,10 @@ def col_mean(x, n_cols=None, stream=None):
     rows, ld = x.shape
     cols = ld if n_cols is None else n_cols
     out = torch.zeros(ld, dtype=torch.float32, device=x.device)
-    _lib.check(lib.mtip_col_mean(_ptr(x), rows, cols, ld, _ptr(out), _stream(stream)),
-               "mtip_col_mean")
+    nb = int(lib.mtip_col_mean_workspace_size(cols))
+    work = torch.empty(max(nb, 8), dtype=torch.uint8, device=x.device)
+    _lib.check(lib.mtip_col_mean(_ptr(x), rows, cols, ld, _ptr(work), work.numel(), _ptr(out),
+                                 _stream(stream)), "mtip_col_mean")
     return out
 
 
@@ -352,7 +354,8 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
                                     _ptr(workspace), workspace.numel(), st)
     _lib.check(rc, "mtip_match_partial")
     with _span(timers, "merge"):
-        rc = lib.mtip_topk_merge(_ptr(workspace), N_o, N_i, K, _ptr(idx), _ptr(val), st)
+        rc = lib.mtip_topk_merge(_lib.PRECISIONS[precision], _ptr(workspace), N_o, N_i, K,
+                                 _ptr(idx), _ptr(val), st)
     _lib.check(rc, "mtip_topk_merge")
     return idx, val
 
