@@ -148,6 +148,18 @@ int mtip_scaling_ratio(const double *img_sum, const double *img_sumsq, int64_t n
                        int64_t n_pix, double *alpha, void *stream);
 
 /*
+ * Per-request scaling ratio: for each segment g of images [seg_offsets[g],
+ * seg_offsets[g+1]) (device int64, n_seg + 1 entries) alpha[i] = std(images
+ * of the segment) / std(slices) for every image i of it (device FP64, N_i).
+ * A batch of independent requests then matches each request exactly as if
+ * it were matched alone.
+ */
+int mtip_segment_scaling_ratio(const double *img_sum, const double *img_sumsq,
+                               const int64_t *seg_offsets, int64_t n_seg, const double *sl_sum,
+                               const double *sl_sumsq, int64_t n_sl, int64_t n_pix, double *alpha,
+                               void *stream);
+
+/*
  * Cast rows to the GEMM operand format.  format MTIP_BF16: out is uint16
  * bf16 (round to nearest even), scales unused (nullable).  format MTIP_FP8:
  * out is uint8 OCP e4m3 with a per-row scale = absmax / 448 written to
@@ -157,8 +169,10 @@ int mtip_quantize_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_i
                        void *out, int64_t ld_out, float *scales, void *stream);
 
 /*
- * Residual rows for the GEMM operands: out_r = cast(m * in_r - center) with
- * m = *mult (or 1 / *mult when invert_mult; 1 when mult is NULL), center a
+ * Residual rows for the GEMM operands: out_r = cast(m_r * in_r - center) with
+ * m_r from the device FP64 array mult: mult[0] for every row, or mult[r]
+ * when bit 1 of mult_flags is set; its inverse when bit 0 is set; 1 when
+ * mult is NULL.  center a
  * device float32 (cols) vector (NULL = 0), format MTIP_FP32 / MTIP_BF16 /
  * MTIP_FP8 (e4m3 with per-row scales absmax / 448).  sumsq (nullable):
  * device FP64 (rows) squared norms of the values as cast.  Distances are
@@ -167,7 +181,7 @@ int mtip_quantize_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_i
  * (the bright low-q region) before the cast.  Columns [cols, ld_out) are 0.
  */
 int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in,
-                       const double *mult, int invert_mult, const float *center, int format,
+                       const double *mult, int mult_flags, const float *center, int format,
                        void *out, int64_t ld_out, float *scales, double *sumsq, void *stream);
 
 /*
@@ -227,6 +241,9 @@ int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N
 /*
  * Rescore K candidates per image on FP32 slices/images:
  *   d(n, i) = sum_p (alpha s_n[p] - d_i[p])^2
+ * with alpha[0] for every image, or alpha[i] per image when
+ * alpha_per_image (a request's own scaling ratio, so that its result does not
+ * depend on which other images share the batch),
  * once with FP32 arithmetic (d32, nullable) and once in FP64 (d64,
  * nullable); best_idx / best_d64 (device, N_i) take the FP64 minimum
  * (lowest orientation index on ties).  cand: device int32 (N_i, K);
@@ -234,7 +251,8 @@ int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N
  * spinifel/sequential/nearest_neighbor.py:134 (squared, before the sqrt).
  */
 int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
-                 int64_t N_i, int64_t N_p, const double *alpha, const int32_t *cand, int K,
+                 int64_t N_i, int64_t N_p, const double *alpha, int alpha_per_image,
+                 const int32_t *cand, int K,
                  int32_t *best_idx, double *best_d64, float *d32, double *d64, void *stream);
 
 /*

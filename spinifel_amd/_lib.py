@@ -44,6 +44,7 @@ _SIGS = {
     "mtip_rotate_positions": (_i32, [_vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "mtip_scaling_ratio": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "mtip_segment_scaling_ratio": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     "mtip_quantize_rows": (_i32, [_vp, _i64, _i64, _i64, _i32, _vp, _i64, _vp, _vp]),
     "mtip_residual_rows": (_i32, [_vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp, _i64, _vp,
                                   _vp, _vp]),
@@ -55,8 +56,8 @@ _SIGS = {
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                                   _vp, _i32, _vp, _c.c_size_t, _vp]),
     "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
-    "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _vp, _i32, _vp, _vp,
-                            _vp, _vp, _vp]),
+    "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp,
+                            _vp, _vp, _vp, _vp]),
     "mtip_euclidean_distance": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),
     "mtip_heap_sort": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp]),
 }

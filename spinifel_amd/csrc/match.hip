@@ -547,6 +547,7 @@ template <int K>
 __global__ __launch_bounds__(256) void k_rescore(const float *__restrict__ S, int64_t lds,
                                                  int64_t N_o, const float *__restrict__ D, int64_t ldd,
                                                  int64_t N_p, const double *__restrict__ alpha,
+                                                 int alpha_per_image,
                                                  const int32_t *__restrict__ cand,
                                                  int32_t *__restrict__ best_idx,
                                                  double *__restrict__ best_d64,
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(256) void k_rescore(const float *__restrict__ S, in
     __shared__ float s32[4][K];
     const int64_t i = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const double al = *alpha;
+    const double al = alpha_per_image ? alpha[i] : alpha[0];
     const float alf = (float)al;
     int32_t c[K];
 #pragma unroll
@@ -676,6 +677,37 @@ __global__ __launch_bounds__(256) void k_scaling_ratio(const double *__restrict_
     }
 }
 
+// alpha per image: for each segment [off[g], off[g+1]) of images (one request),
+// std(images of the segment) / std(all slices), written to every image of it.
+__global__ __launch_bounds__(256) void k_segment_ratio(const double *__restrict__ is,
+                                                       const double *__restrict__ iq,
+                                                       const int64_t *__restrict__ off,
+                                                       const double *__restrict__ ss,
+                                                       const double *__restrict__ sq, int64_t ns,
+                                                       int64_t n_pix, double *__restrict__ alpha)
+{
+    __shared__ double red[4][4];
+    const int64_t g = blockIdx.x;
+    const int64_t a0 = off[g], a1 = off[g + 1];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int64_t k = a0 + threadIdx.x; k < a1; k += 256) { a += is[k]; b += iq[k]; }
+    for (int64_t k = threadIdx.x; k < ns; k += 256) { c += ss[k]; d += sq[k]; }
+    a = wave_sum_f64(a);
+    b = wave_sum_f64(b);
+    c = wave_sum_f64(c);
+    d = wave_sum_f64(d);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w][0] = a; red[w][1] = b; red[w][2] = c; red[w][3] = d; }
+    __syncthreads();
+    double t[4];
+    for (int k = 0; k < 4; ++k) t[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    const double Ni = (double)(a1 - a0) * (double)n_pix, Ns = (double)ns * (double)n_pix;
+    const double mi = t[0] / Ni, ms = t[2] / Ns;
+    const double vi = fmax(t[1] / Ni - mi * mi, 0.0), vs = fmax(t[3] / Ns - ms * ms, 0.0);
+    const double r = sqrt(vi) / sqrt(vs);
+    for (int64_t k = a0 + threadIdx.x; k < a1; k += 256) alpha[k] = r;
+}
+
 __device__ __forceinline__ uint16_t bf16_rne(float f)
 {
     uint32_t u = __float_as_uint(f);
@@ -712,7 +744,10 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ in, 
     const int64_t r = blockIdx.x;
     const float *src = in + r * ld_in;
     float m = 1.0f;
-    if (mult) m = invert ? (float)(1.0 / *mult) : (float)(*mult);
+    if (mult) {
+        const double mv = (invert & 2) ? mult[r] : mult[0];
+        m = (invert & 1) ? (float)(1.0 / mv) : (float)mv;
+    }
     float scale = 1.0f, inv = 1.0f;
     if (FMT == MTIP_FP8) {
         float a = 0.0f;
@@ -968,7 +1003,8 @@ extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
 }
 
 extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
-                            int64_t N_i, int64_t N_p, const double *alpha, const int32_t *cand,
+                            int64_t N_i, int64_t N_p, const double *alpha, int alpha_per_image,
+                            const int32_t *cand,
                             int K, int32_t *best_idx, double *best_d64, float *d32, double *d64,
                             void *stream)
 {
@@ -981,10 +1017,10 @@ extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, cons
     hipStream_t st = (hipStream_t)stream;
     dim3 g((unsigned)N_i);
     switch (K) {
-    case 1: hipLaunchKernelGGL(k_rescore<1>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
-    case 2: hipLaunchKernelGGL(k_rescore<2>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
-    case 4: hipLaunchKernelGGL(k_rescore<4>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
-    default: hipLaunchKernelGGL(k_rescore<8>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, cand, best_idx, best_d64, d32, d64); break;
+    case 1: hipLaunchKernelGGL(k_rescore<1>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
+    case 2: hipLaunchKernelGGL(k_rescore<2>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
+    case 4: hipLaunchKernelGGL(k_rescore<4>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
+    default: hipLaunchKernelGGL(k_rescore<8>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
     }
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
@@ -1017,7 +1053,7 @@ extern "C" int mtip_scaling_ratio(const double *img_sum, const double *img_sumsq
 }
 
 extern "C" int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, int64_t ld_in,
-                                  const double *mult, int invert_mult, const float *center,
+                                  const double *mult, int mult_flags, const float *center,
                                   int format, void *out, int64_t ld_out, float *scales,
                                   double *sumsq, void *stream)
 {
@@ -1030,14 +1066,14 @@ extern "C" int mtip_residual_rows(const float *in, int64_t rows, int64_t cols, i
     dim3 g((unsigned)rows);
     if (format == MTIP_FP32)
         hipLaunchKernelGGL(k_residual<MTIP_FP32>, g, dim3(256), 0, st, in, cols, ld_in, mult,
-                           invert_mult, center, out, ld_out, scales, sumsq);
+                           mult_flags, center, out, ld_out, scales, sumsq);
     else if (format == MTIP_BF16)
         hipLaunchKernelGGL(k_residual<MTIP_BF16>, g, dim3(256), 0, st, in, cols, ld_in, mult,
-                           invert_mult, center, out, ld_out, scales, sumsq);
+                           mult_flags, center, out, ld_out, scales, sumsq);
     else if (format == MTIP_FP8) {
         MTIP_REQUIRE(scales, MTIP_ERR_ARG, "mtip_residual_rows: FP8 needs a scales array");
         hipLaunchKernelGGL(k_residual<MTIP_FP8>, g, dim3(256), 0, st, in, cols, ld_in, mult,
-                           invert_mult, center, out, ld_out, scales, sumsq);
+                           mult_flags, center, out, ld_out, scales, sumsq);
     } else {
         return mtip::fail(MTIP_ERR_ARG, "mtip_residual_rows: unknown format");
     }
@@ -1117,6 +1153,22 @@ extern "C" int mtip_heap_sort(const float *euDist, int64_t numData, int64_t numR
         launch_topk_rows<4>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
     else
         launch_topk_rows<8>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_segment_scaling_ratio(const double *img_sum, const double *img_sumsq,
+                                          const int64_t *seg_offsets, int64_t n_seg,
+                                          const double *sl_sum, const double *sl_sumsq, int64_t n_sl,
+                                          int64_t n_pix, double *alpha, void *stream)
+{
+    MTIP_REQUIRE(n_seg >= 0 && n_sl > 0 && n_pix > 0, MTIP_ERR_ARG, "mtip_segment_scaling_ratio: bad size");
+    if (n_seg == 0) return MTIP_OK;
+    MTIP_REQUIRE(n_seg <= 0x7fffffff, MTIP_ERR_ARG, "mtip_segment_scaling_ratio: too many segments");
+    MTIP_REQUIRE(img_sum && img_sumsq && seg_offsets && sl_sum && sl_sumsq && alpha, MTIP_ERR_ARG,
+                 "mtip_segment_scaling_ratio: null pointer");
+    hipLaunchKernelGGL(k_segment_ratio, dim3((unsigned)n_seg), dim3(256), 0, (hipStream_t)stream,
+                       img_sum, img_sumsq, seg_offsets, sl_sum, sl_sumsq, n_sl, n_pix, alpha);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }

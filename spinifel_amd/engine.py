@@ -258,6 +258,19 @@ def scaling_ratio(img_sum, img_sumsq, sl_sum, sl_sumsq, n_pix, stream=None):
     return alpha
 
 
+def segment_scaling_ratio(img_sum, img_sumsq, offsets, sl_sum, sl_sumsq, n_pix, stream=None):
+    """alpha per image: each segment (request) scaled by its own image std."""
+    lib = _lib.load()
+    dev = img_sum.device
+    off = torch.as_tensor(np.asarray(offsets, dtype=np.int64)).to(dev)
+    alpha = torch.empty(img_sum.numel(), dtype=torch.float64, device=dev)
+    _lib.check(lib.mtip_segment_scaling_ratio(_ptr(img_sum), _ptr(img_sumsq), _ptr(off),
+                                              off.numel() - 1, _ptr(sl_sum), _ptr(sl_sumsq),
+                                              sl_sum.numel(), int(n_pix), _ptr(alpha),
+                                              _stream(stream)), "mtip_segment_scaling_ratio")
+    return alpha
+
+
 @dataclass
 class Operand:
     """A row matrix resident in one GEMM operand format: data (rows, ld),
@@ -281,7 +294,8 @@ def residual(x, precision, n_cols=None, mult=None, invert=False, center=None, st
     out = torch.empty((rows, ld), dtype=dt, device=x.device)
     sc = torch.empty(rows, dtype=torch.float32, device=x.device) if precision == "fp8" else None
     ss = torch.empty(rows, dtype=torch.float64, device=x.device)
-    _lib.check(lib.mtip_residual_rows(_ptr(x), rows, cols, ld, _ptr(mult), int(bool(invert)),
+    flags = (1 if invert else 0) | (2 if (mult is not None and mult.numel() > 1) else 0)
+    _lib.check(lib.mtip_residual_rows(_ptr(x), rows, cols, ld, _ptr(mult), flags,
                                       _ptr(center), code, _ptr(out), ld, _ptr(sc), _ptr(ss),
                                       _stream(stream)), "mtip_residual_rows")
     return Operand(out, sc, ss)
@@ -371,7 +385,8 @@ def rescore(S32, D32, N_p, alpha, cand, stream=None, timers=None):
     d64 = torch.empty((N_i, K), dtype=torch.float64, device=dev)
     with _span(timers, "rescore"):
         rc = lib.mtip_rescore(_ptr(S32), S32.shape[1], S32.shape[0], _ptr(D32), D32.shape[1], N_i,
-                              N_p, _ptr(alpha), _ptr(cand.contiguous()), K, _ptr(best), _ptr(bd),
+                              N_p, _ptr(alpha), int(alpha.numel() > 1), _ptr(cand.contiguous()), K,
+                              _ptr(best), _ptr(bd),
                               _ptr(d32), _ptr(d64), _stream(stream))
     _lib.check(rc, "mtip_rescore")
     return dict(best=best, best_d64=bd, d32=d32, d64=d64)

@@ -1,0 +1,292 @@
+"""Serving form: resident models, continuous batching of image requests.
+
+A ``Server`` keeps several models resident in HBM, each with its reference
+slices computed once (FP32 for rescoring plus the centred residual in the
+GEMM format, bf16 by default).  Requests (a batch of images tagged with a
+model) are queued; a scheduler thread admits them onto ``n_slots`` HIP
+streams.  Whenever a slot is free it takes the oldest pending request and
+every other pending request for the same model (up to ``max_batch_images``),
+provided the device has enough free HBM for the batch (``mtip_mem_info``);
+requests that arrive while others run are therefore batched as soon as a
+slot frees up, and their work overlaps with the other slots on the device.
+
+Each request is scaled by its own std ratio (``mtip_segment_scaling_ratio``)
+and every per-image quantity (cross term, top-K keys, FP64 rescore) depends
+only on that image, so a request's result is the same alone and in any mix.
+"""
+import ctypes
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, engine
+
+
+@dataclass
+class ServedModel:
+    name: str
+    ref_quats: np.ndarray
+    slices: engine.SliceSet
+    operand: engine.Operand
+    N_p: int
+    det_shape: tuple
+
+
+@dataclass(eq=False)
+class Request:
+    rid: int
+    model: str
+    images: np.ndarray
+    t_submit: float
+    future: Future = field(default_factory=Future)
+    t_start: float = 0.0
+    t_done: float = 0.0
+    batch_size: int = 0
+
+
+@dataclass
+class Result:
+    orientations: np.ndarray
+    index: np.ndarray
+    dist64: np.ndarray
+    latency_s: float
+    batch_images: int
+
+
+def free_hbm(device=0):
+    lib = _lib.load()
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _lib.check(lib.mtip_mem_info(int(device), ctypes.byref(f), ctypes.byref(t)), "mtip_mem_info")
+    return f.value, t.value
+
+
+class Server:
+    def __init__(self, precision="bf16", K=8, n_slots=4, max_batch_images=4096,
+                 mem_reserve_bytes=8 << 30, device=0, eps=1e-6):
+        engine.require_gpu()
+        self.precision = precision
+        self.K = K
+        self.device = torch.device("cuda", device)
+        self.n_slots = n_slots
+        self.max_batch_images = max_batch_images
+        self.mem_reserve = mem_reserve_bytes
+        self.eps = eps
+        self.models = {}
+        self._plans = {}
+        self._queue = deque()
+        self._lock = threading.Condition()
+        self._streams = [torch.cuda.Stream(device=self.device) for _ in range(n_slots)]
+        self._busy = [None] * n_slots          # (requests, event, outputs) per slot
+        self._next_id = 0
+        self.error = None
+        self._stop = False
+        self._thread = None
+        self.completed = []
+
+    # ------------------------------------------------------------- models
+    def add_model(self, name, ac, ref_quats, pixel_position_reciprocal, pixel_distance_reciprocal,
+                  oversampling=1):
+        ac = np.asarray(ac, np.float32)
+        M = ac.shape[0]
+        key = (M, self.eps)
+        if key not in self._plans:
+            self._plans[key] = engine.NufftPlan(M, self.eps, self.device.index or 0)
+        plan = self._plans[key]
+        plan.set_model(ac)
+        ppr = np.asarray(pixel_position_reciprocal, np.float64)
+        E = float(np.asarray(pixel_distance_reciprocal).max())
+        sl = engine.SliceSet.compute(plan, np.asarray(ref_quats, np.float64), ppr.reshape(3, -1),
+                                     np.pi / (E * oversampling),
+                                     order=engine.tile_order(ppr.shape[1:]))
+        torch.cuda.synchronize(self.device)
+        if int(sl.bad.item()) > 0:
+            raise AssertionError("NU points outside [-3pi, 3pi)")
+        op = sl.operand(self.precision, True)
+        self.models[name] = ServedModel(name, np.asarray(ref_quats), sl, op, sl.N_p, ppr.shape[1:])
+        return self.models[name]
+
+    # ------------------------------------------------------------- requests
+    def submit(self, model, images):
+        if model not in self.models:
+            raise KeyError(model)
+        images = np.ascontiguousarray(images, np.float32).reshape(len(images), -1)
+        if images.shape[1] != self.models[model].N_p:
+            raise ValueError(f"images have {images.shape[1]} pixels, model {model} "
+                             f"{self.models[model].N_p}")
+        with self._lock:
+            r = Request(self._next_id, model, images, time.perf_counter())
+            self._next_id += 1
+            self._queue.append(r)
+            self._lock.notify()
+        return r.future
+
+    def match_now(self, model, images):
+        """Synchronous single-request match (no queue), the 'alone' reference."""
+        r = Request(-1, model, np.ascontiguousarray(images, np.float32).reshape(len(images), -1),
+                    time.perf_counter())
+        out = self._launch([r], torch.cuda.current_stream(self.device))
+        torch.cuda.synchronize(self.device)
+        return self._collect([r], out)[0]
+
+    # ------------------------------------------------------------- engine
+    def _batch_bytes(self, n_img, N_p):
+        per = N_p * (4 + 4 + 2) + 64 * self.K
+        return n_img * per + (64 << 20)
+
+    def _launch(self, reqs, stream):
+        m = self.models[reqs[0].model]
+        sizes = [len(r.images) for r in reqs]
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        host = np.concatenate([r.images for r in reqs]) if len(reqs) > 1 else reqs[0].images
+        with torch.cuda.stream(stream):
+            imgs = engine.ImageSet(torch.from_numpy(host).pin_memory().to(self.device, non_blocking=True),
+                                   self.device, stream)
+            alpha = engine.segment_scaling_ratio(imgs.row_sum, imgs.row_sumsq, offs,
+                                                 m.slices.row_sum, m.slices.row_sumsq, m.N_p, stream)
+            D = imgs.operand(self.precision, alpha, m.slices.center, stream)
+            one = engine._one(self.device)
+            idx, val = engine.topk(self.precision, m.operand, m.operand.sumsq, m.slices.N_o, D,
+                                   imgs.N_i, imgs.N_p, one, self.K, None, stream)
+            rs = engine.rescore(m.slices.f32, imgs.f32, imgs.N_p, alpha, idx, stream)
+            best = rs["best"].to("cpu", non_blocking=True)
+            bd = rs["best_d64"].to("cpu", non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return dict(best=best, bd=bd, event=ev, offs=offs, keep=(imgs, D, idx, rs))
+
+    def _collect(self, reqs, out):
+        now = time.perf_counter()
+        best = out["best"].numpy().astype(np.int64)
+        bd = out["bd"].numpy()
+        res = []
+        for r, a, b in zip(reqs, out["offs"][:-1], out["offs"][1:]):
+            m = self.models[r.model]
+            ix = best[a:b]
+            res.append(Result(m.ref_quats[ix], ix, bd[a:b], now - r.t_submit, int(out["offs"][-1])))
+        return res
+
+    # ------------------------------------------------------------- scheduler
+    def start(self):
+        self._stop = False
+        self._thread = threading.Thread(target=self._loop, daemon=True)
+        self._thread.start()
+
+    def stop(self):
+        with self._lock:
+            self._stop = True
+            self._lock.notify()
+        if self._thread:
+            self._thread.join()
+
+    def _take_batch(self):
+        """Oldest pending request plus the other pending ones of its model."""
+        if not self._queue:
+            return None
+        first = self._queue[0]
+        m = self.models[first.model]
+        batch, n = [], 0
+        for r in list(self._queue):
+            if r.model != first.model:
+                continue
+            if batch and n + len(r.images) > self.max_batch_images:
+                break
+            batch.append(r)
+            n += len(r.images)
+        free, _ = free_hbm(self.device.index or 0)
+        if free - self.mem_reserve < self._batch_bytes(n, m.N_p):
+            return None                      # wait for memory to free up
+        for r in batch:
+            self._queue.remove(r)
+        return batch
+
+    def _loop(self):
+        try:
+            self._loop_body()
+        except BaseException as e:          # surface scheduler failures to every waiter
+            self.error = e
+            with self._lock:
+                pending = list(self._queue)
+                self._queue.clear()
+            for job in self._busy:
+                if job is not None:
+                    pending.extend(job[0])
+            for r in pending:
+                if not r.future.done():
+                    r.future.set_exception(e)
+
+    def _loop_body(self):
+        torch.cuda.set_device(self.device)
+        while True:
+            progressed = False
+            # retire finished slots
+            for s in range(self.n_slots):
+                job = self._busy[s]
+                if job is not None and job[1]["event"].query():
+                    reqs, out = job
+                    for r, res in zip(reqs, self._collect(reqs, out)):
+                        r.t_done = time.perf_counter()
+                        r.batch_size = res.batch_images
+                        self.completed.append(r)
+                        r.future.set_result(res)
+                    self._busy[s] = None
+                    progressed = True
+            # admit new batches onto free slots
+            with self._lock:
+                if self._stop and not self._queue and all(b is None for b in self._busy):
+                    return
+                for s in range(self.n_slots):
+                    if self._busy[s] is None:
+                        batch = self._take_batch()
+                        if batch is None:
+                            break
+                        t = time.perf_counter()
+                        for r in batch:
+                            r.t_start = t
+                        self._busy[s] = (batch, self._launch(batch, self._streams[s]))
+                        progressed = True
+                if not progressed:
+                    self._lock.wait(timeout=0.0005)
+
+    # ------------------------------------------------------------- stats
+    def stats(self):
+        lat = np.array([r.t_done - r.t_submit for r in self.completed])
+        n_img = sum(len(r.images) for r in self.completed)
+        if not len(lat):
+            return {}
+        t0 = min(r.t_submit for r in self.completed)
+        t1 = max(r.t_done for r in self.completed)
+        return dict(requests=len(lat), images=n_img, p50_ms=1e3 * float(np.percentile(lat, 50)),
+                    p99_ms=1e3 * float(np.percentile(lat, 99)), mean_ms=1e3 * float(lat.mean()),
+                    sustained_images_per_s=n_img / (t1 - t0),
+                    mean_batch_images=float(np.mean([r.batch_size for r in self.completed])))
+
+
+def poisson_bursts(rate_hz, duration_s, max_burst, models, rng):
+    """Seeded arrival schedule: (time offset, model, burst size) per burst."""
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t > duration_s:
+            return out
+        out.append((t, models[int(rng.integers(0, len(models)))], int(rng.integers(1, max_burst + 1))))
+
+
+def run_load(server, schedule, image_source):
+    """Replay an arrival schedule in real time; image_source(model, n) -> images."""
+    server.start()
+    t0 = time.perf_counter()
+    futs = []
+    for (dt, model, n) in schedule:
+        imgs = image_source(model, n)
+        while time.perf_counter() - t0 < dt:
+            time.sleep(min(0.0005, max(0.0, dt - (time.perf_counter() - t0))))
+        futs.append(server.submit(model, imgs))
+    for f in futs:
+        f.result()
+    server.stop()
+    return server.stats()

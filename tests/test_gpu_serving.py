@@ -1,0 +1,87 @@
+"""Serving form on the GPU: continuous batching gives each request the same
+result alone and in the mix, and matches the oracle."""
+import numpy as np
+import pytest
+
+from helpers import oracle_slicer
+from oracle import matching
+from spinifel_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(M, atoms, seed, n_ref, panels=1, rows=32, cols=32):
+    ac = synth.make_model(M, atoms, seed)
+    ppr, pdr = synth.detector(panels, rows, cols)
+    ref = synth.random_quaternions(n_ref, np.random.default_rng(seed + 100))
+    return ac, ppr, pdr, ref
+
+
+@pytest.fixture(scope="module")
+def server():
+    from spinifel_amd import serving
+    srv = serving.Server(precision="bf16", K=8, n_slots=3, max_batch_images=600)
+    models = {}
+    for name, (M, atoms, seed, n_ref) in {"a": (33, 120, 1, 1200), "b": (41, 300, 2, 2000)}.items():
+        ac, ppr, pdr, ref = _model(M, atoms, seed, n_ref)
+        srv.add_model(name, ac, ref, ppr, pdr)
+        true = synth.random_quaternions(400, np.random.default_rng(seed + 7))
+        sl = oracle_slicer(1e-9)(ac, true, ppr, pdr.max())
+        imgs = synth.poisson_images(sl, 0.5, np.random.default_rng(seed + 9))
+        models[name] = dict(ac=ac, ppr=ppr, pdr=pdr, ref=ref, imgs=imgs)
+    return srv, models
+
+
+def test_requests_alone_equal_in_mix(server):
+    srv, models = server
+    rng = np.random.default_rng(0)
+    reqs = []
+    for _ in range(24):
+        name = "ab"[int(rng.integers(0, 2))]
+        n = int(rng.integers(1, 90))
+        sel = rng.choice(400, n, replace=False)
+        reqs.append((name, models[name]["imgs"][sel]))
+    alone = [srv.match_now(name, imgs) for name, imgs in reqs]
+    srv.start()
+    futs = [srv.submit(name, imgs) for name, imgs in reqs]
+    mixed = [f.result(timeout=120) for f in futs]
+    srv.stop()
+    batched = sum(1 for r in mixed if r.batch_images > 0)
+    assert batched == len(reqs)
+    assert max(r.batch_images for r in mixed) > max(len(i) for _, i in reqs), "no batching happened"
+    for a, m in zip(alone, mixed):
+        np.testing.assert_array_equal(a.index, m.index)
+        np.testing.assert_array_equal(a.dist64, m.dist64)     # bitwise: per-image arithmetic only
+    st = srv.stats()
+    assert st["requests"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_served_result_matches_oracle(server):
+    srv, models = server
+    m = models["b"]
+    imgs = m["imgs"][:200]
+    got = srv.match_now("b", imgs)
+    ora = matching.slicing_and_match(m["ac"], imgs.reshape((200, 1, 32, 32)), m["ppr"], m["pdr"],
+                                     m["ref"], return_details=True)
+    assert (got.index == ora["index"]).mean() == 1.0
+    # FP64 distance of the winner equals the oracle's squared distance
+    np.testing.assert_allclose(got.dist64, ora["min_dist"] ** 2, rtol=1e-5)
+
+
+def test_admission_waits_for_memory(server):
+    """With the reserve set above the free HBM nothing is admitted; lowering it
+    releases the queue (the host-side admission rule, forced from the host)."""
+    from spinifel_amd import serving
+    srv, models = server
+    free, _ = serving.free_hbm(0)
+    old = srv.mem_reserve
+    srv.mem_reserve = free + (1 << 40)
+    srv.start()
+    fut = srv.submit("a", models["a"]["imgs"][:10])
+    import time
+    time.sleep(0.3)
+    assert not fut.done()
+    srv.mem_reserve = old
+    res = fut.result(timeout=60)
+    srv.stop()
+    assert len(res.index) == 10
