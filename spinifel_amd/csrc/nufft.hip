@@ -14,6 +14,8 @@
 
 #include "mtip_internal.h"
 
+#include <type_traits>
+
 struct mtip_plan {
     int device = 0;
     int M = 0, h = 0, K = 0;      // model edge, half width, symmetric extent 2h+1
@@ -187,6 +189,7 @@ __global__ __launch_bounds__(256) void k_dft_pass(const float *__restrict__ ac,
 // results with xor-shuffles.
 
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
+typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
 
 struct GridView {
     const float *__restrict__ g;
@@ -424,55 +427,33 @@ __global__ __launch_bounds__(256) void k_interp(GridView gv, const double *__res
 
 // ------------------------------------------------------------ LDS-staged slicing
 //
-// One workgroup per orientation; each wave takes 64-pixel tiles (8 x 8
-// detector pixels when the caller passes a tile order).  A tile's points lie
+// One workgroup per orientation walks the detector in tiles of kTile pixels
+// (16 x 16 blocks when the caller passes a tile order).  A tile's points lie
 // on a small patch of the rotated Ewald sphere, so the union of their w^3
-// windows fits a small box of the fine grid (about 2k floats for the 8 x 8
-// tiles of a 128 x 128 detector at 0.9 Nyquist): the wave stages that box in
-// LDS with coalesced row loads (several box rows per instruction), then runs
-// the taps from LDS, lane c of each G-lane group owning tap c of the fast
-// axis, with compile-time row offsets.  A tile whose box exceeds the per-wave
-// LDS capacity uses direct buffer loads from the grid instead.
+// windows fits a small box of the fine grid (about 8k floats at M = 81, 0.9
+// Nyquist): the workgroup stages that box in LDS with 16-byte loads, then
+// runs the taps from LDS.  GP lanes serve one point: lane c owns the pair of
+// fast-axis elements (2c, 2c + 1) counted from the even element at or below
+// the window start, so each LDS read is one aligned 8-byte read and elements
+// outside the window get a zero weight.  The other two axes' weights move
+// between the group's lanes by DPP broadcasts.  A tile whose box exceeds
+// kBoxCap reads the grid directly (well under 1 % of tiles at M = 81).
 
-constexpr int kBoxCap = 10240;    // floats of LDS per workgroup for the staged box (40 KiB)
-constexpr int kTile = 256;        // pixels per workgroup tile (16 x 16 detector block)
-
-template <int W, int G, int BZP>
-__device__ __forceinline__ float taps_lds(const float *__restrict__ box, int lbase, int by,
-                                          const float (&wa)[W], const float (&wb)[W])
-{
-    float acc = 0.f;
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-        const float *pa = box + lbase + a * by * BZP;
-        float tab = 0.f;
-#pragma unroll
-        for (int b = 0; b < W; ++b) tab = fmaf(wb[b], pa[b * BZP], tab);
-        acc = fmaf(wa[a], tab, acc);
-    }
-    return acc;
-}
+// 47 KiB of box: with the point arrays a workgroup takes under 53.4 KiB of
+// LDS, so three fit the 160 KiB of a CU.
+constexpr int kBoxCap = 12032;
+constexpr int kTile = 256;        // pixels per workgroup tile
+// 16-byte staging chunks per thread loaded in one batch: a typical box at
+// M = 81 needs about 8, the largest within kBoxCap up to 15 (the rest follow
+// in batches of 4; a larger first batch costs registers and occupancy)
+constexpr int kStageChunks = 8;
 
 template <int W>
-__device__ __forceinline__ float taps_direct(__amdgpu_buffer_rsrc_t rsrc, int np, int voff,
-                                             const float (&wa)[W], const float (&wb)[W])
-{
-    uint32_t raw[W][W];
-#pragma unroll
-    for (int a = 0; a < W; ++a)
-#pragma unroll
-        for (int b = 0; b < W; ++b)
-            raw[a][b] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (a * np * np + b * np) * 4, 0);
-    float acc = 0.f;
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-        float tab = 0.f;
-#pragma unroll
-        for (int b = 0; b < W; ++b) tab = fmaf(wb[b], __uint_as_float(raw[a][b]), tab);
-        acc = fmaf(wa[a], tab, acc);
-    }
-    return acc;
-}
+struct PairTaps {
+    static constexpr int NP = (W + 2) / 2;                                  // pairs per window
+    static constexpr int GP = NP <= 2 ? 2 : NP <= 4 ? 4 : NP <= 8 ? 8 : 16;  // lanes per point
+    static constexpr int NS = (W + GP - 1) / GP;                            // weights per lane
+};
 
 // Sum over aligned groups of G lanes; the result is valid in lane 0 of each
 // group.  DPP moves (VALU) instead of LDS permutes: quad_perm for xor 1 / 2,
@@ -493,16 +474,99 @@ __device__ __forceinline__ float group_sum(float v)
     return v;
 }
 
+// Value of lane K of this lane's aligned group of GP lanes.
+template <int GP, int K>
+__device__ __forceinline__ float group_bcast(float x, int lane)
+{
+    if constexpr (GP == 2)
+        return dpp_f<K | (K << 2) | ((K + 2) << 4) | ((K + 2) << 6)>(x);   // quad_perm [K,K,K+2,K+2]
+    else if constexpr (GP == 4)
+        return dpp_f<0x55 * K>(x);                                         // quad_perm [K,K,K,K]
+    else
+        return __shfl(x, (lane & ~(GP - 1)) + K, 64);
+}
+
+// w[a] = weight a of the group's point; lane c holds weights c, c + GP, ...
+template <int GP, int W, int A = 0>
+__device__ __forceinline__ void gather_weights(const float (&own)[PairTaps<W>::NS], float (&w)[W],
+                                               int lane)
+{
+    if constexpr (A < W) {
+        w[A] = group_bcast<GP, A % GP>(own[A / GP], lane);
+        gather_weights<GP, W, A + 1>(own, w, lane);
+    }
+}
+
+// Taps of one lane's element pair from the staged box: rows b of plane a at
+// compile-time offsets b * S, planes at runtime offsets a * plane.
+constexpr int kPlanesInFlight = 2;
+
+template <int W, int S>
+__device__ __forceinline__ void taps_pair_lds(const float *__restrict__ box, int base, int plane,
+                                              const float (&wa)[W], const float (&wb)[W],
+                                              float &r0, float &r1)
+{
+    float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        const float2 *pa = reinterpret_cast<const float2 *>(box + base + a * plane);
+        float t0 = 0.f, t1 = 0.f;
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const float2 v = pa[b * (S / 2)];
+            t0 = fmaf(wb[b], v.x, t0);
+            t1 = fmaf(wb[b], v.y, t1);
+        }
+        a0 = fmaf(wa[a], t0, a0);
+        a1 = fmaf(wa[a], t1, a1);
+        // keep the reads of at most kPlanesInFlight planes ahead of their FMAs:
+        // hoisting all W^2 reads costs 2 W^2 registers and occupancy
+        if ((a + 1) % kPlanesInFlight == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    r0 = a0;
+    r1 = a1;
+}
+
+// The same taps read from the padded grid (voff = byte offset of the pair).
+template <int W>
+__device__ __forceinline__ void taps_pair_direct(__amdgpu_buffer_rsrc_t rsrc, int np, int voff,
+                                                 const float (&wa)[W], const float (&wb)[W],
+                                                 float &r0, float &r1)
+{
+    // planes in two halves, each half's loads issued before its FMAs
+    constexpr int HA = (W + 1) / 2;
+    float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+    for (int a_lo = 0; a_lo < W; a_lo += HA) {
+        uint2v raw[HA][W];
+#pragma unroll
+        for (int i = 0; i < HA; ++i)
+#pragma unroll
+            for (int b = 0; b < W; ++b)
+                if (a_lo + i < W)
+                    raw[i][b] = __builtin_amdgcn_raw_buffer_load_b64(
+                        rsrc, voff, ((a_lo + i) * np * np + b * np) * 4, 0);
+#pragma unroll
+        for (int i = 0; i < HA; ++i) {
+            if (a_lo + i >= W) break;
+            float t0 = 0.f, t1 = 0.f;
+#pragma unroll
+            for (int b = 0; b < W; ++b) {
+                t0 = fmaf(wb[b], __uint_as_float(raw[i][b][0]), t0);
+                t1 = fmaf(wb[b], __uint_as_float(raw[i][b][1]), t1);
+            }
+            a0 = fmaf(wa[a_lo + i], t0, a0);
+            a1 = fmaf(wa[a_lo + i], t1, a1);
+        }
+    }
+    r0 = a0;
+    r1 = a1;
+}
+
 __device__ __forceinline__ int wave_min_i(int v)
 {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
     return v;
 }
 
@@ -527,17 +591,20 @@ __device__ __forceinline__ bool point_corner(const GridView &gv, double X0, doub
     return true;
 }
 
-// Odd LDS row strides the tap loop is compiled for.
-#define MTIP_STRIDE_CASES(X) X(17) X(21) X(25) X(29) X(33) X(41) X(49) X(57) X(65)
+// Even LDS row strides the tap loop is compiled for.
+#define MTIP_PAIR_STRIDES(X)                                                                  \
+    X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(36) X(40)  \
+    X(48) X(56) X(64)
 
-__device__ __forceinline__ int odd_stride(int bz)
+__device__ __forceinline__ int pair_stride(int bz)
 {
-    return bz <= 17 ? 17 : bz <= 21 ? 21 : bz <= 25 ? 25 : bz <= 29 ? 29 : bz <= 33 ? 33
-         : bz <= 41 ? 41 : bz <= 49 ? 49 : bz <= 57 ? 57 : bz <= 65 ? 65 : 0;
+    bz += bz & 1;
+    return bz <= 8 ? 8 : bz <= 32 ? bz : bz <= 36 ? 36 : bz <= 40 ? 40 : bz <= 48 ? 48
+         : bz <= 56 ? 56 : bz <= 64 ? 64 : 0;
 }
 
-template <int W, int G>
-__global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *__restrict__ quats,
+template <int W>
+__global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double *__restrict__ quats,
                                                     const double *__restrict__ q, int64_t N_p,
                                                     const int32_t *__restrict__ order,
                                                     int64_t n_order, double gscale,
@@ -546,19 +613,30 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
                                                     double *__restrict__ row_sum,
                                                     double *__restrict__ row_sumsq, int *bad)
 {
-    constexpr int PPW = 64 / G;
-    __shared__ float s_box[kBoxCap];
-    __shared__ int4 s_ijk[kTile];      // window corner (padded grid) + pixel index / flag
-    __shared__ float4 s_t[kTile];
-    __shared__ float s_w[4][PPW][2][G];
+    using PT = PairTaps<W>;
+    constexpr int GP = PT::GP, NS = PT::NS, PPW = 64 / GP;
+    __shared__ __align__(16) float s_box[kBoxCap];
+    // per point: pair-base offset (box-local when staged, padded grid otherwise)
+    // with the window parity in bit 0, or -1 for a point outside the domain
+    __shared__ int s_lb[kTile];
+    __shared__ int s_pix[kTile];       // output pixel, -1 for an empty slot
+    __shared__ float s_t[3][kTile];    // fractional window offsets
     __shared__ int s_red_i[4][6];
     __shared__ double s_red[8];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int g = lane / G, c = lane % G;
+    const int g = lane / GP, c = lane % GP;
     const int64_t o = blockIdx.x;
-    double ri[3][3];
-    quat_inv_rot(quats + 4 * o, ri);
+    // R^-1 of this orientation, kept in LDS: as registers its nine doubles
+    // would be live across the whole kernel
+    __shared__ double s_ri[9];
+    if (tid == 0) {
+        double ri[3][3];
+        quat_inv_rot(quats + 4 * o, ri);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) s_ri[e] = ri[e / 3][e % 3];
+    }
+    __syncthreads();
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)gv.g, (short)0, (int)gv.bytes, 0x00020000);
     const int np = gv.np;
@@ -596,17 +674,19 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
             qz_n = q[2 * N_p + pp];
         }
         pix_nn = pixel_at(tb + 2 * kTile + tid);
-        int ok = 0;
+        bool ok = false;
         int ijk[3] = {0, 0, 0};
         float t[3] = {0.f, 0.f, 0.f};
         if (pix >= 0) {
-            double X0 = (ri[0][0] * x + ri[0][1] * y + ri[0][2] * z) * gscale;
-            double X1 = (ri[1][0] * x + ri[1][1] * y + ri[1][2] * z) * gscale;
-            double X2 = (ri[2][0] * x + ri[2][1] * y + ri[2][2] * z) * gscale;
-            ok = point_corner<W>(gv, X0, X1, X2, ijk, t) ? 1 : 0;
+            double X0 = (s_ri[0] * x + s_ri[1] * y + s_ri[2] * z) * gscale;
+            double X1 = (s_ri[3] * x + s_ri[4] * y + s_ri[5] * z) * gscale;
+            double X2 = (s_ri[6] * x + s_ri[7] * y + s_ri[8] * z) * gscale;
+            ok = point_corner<W>(gv, X0, X1, X2, ijk, t);
         }
-        s_ijk[tid] = make_int4(ijk[0], ijk[1], ijk[2], ok ? pix : (pix >= 0 ? -2 : -1));
-        s_t[tid] = make_float4(t[0], t[1], t[2], 0.f);
+        s_pix[tid] = pix;
+        s_t[0][tid] = t[0];
+        s_t[1][tid] = t[1];
+        s_t[2][tid] = t[2];
         // ---- box of the tile's windows (block min / max)
         {
             int v[6] = {ok ? ijk[0] : 0x7fffffff, ok ? ijk[1] : 0x7fffffff, ok ? ijk[2] : 0x7fffffff,
@@ -626,100 +706,112 @@ __global__ __launch_bounds__(256) void k_slices_lds(GridView gv, const double *_
                          min(s_red_i[2][e + 3], s_red_i[3][e + 3]));
         }
         const bool any = lo[0] != 0x7fffffff;
+        const int lz = lo[2] & ~1;      // even, so every pair base is 8-byte aligned
         const int bx = any ? hi[0] - lo[0] + W : 0, by = any ? hi[1] - lo[1] + W : 0;
-        const int bz = any ? hi[2] - lo[2] + G : 0;
-        const int S = odd_stride(bz);
-        const int lpr = bz <= 16 ? 16 : (bz <= 32 ? 32 : 64);
+        const int bz = any ? hi[2] - lz + 2 * GP : 0;
+        const int S = pair_stride(bz);
         const bool staged = any && S > 0 && bx * by * S <= kBoxCap;
+        {
+            int lb = -1;
+            if (ok) {
+                const int zb = ijk[2] & ~1;
+                lb = staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * S + (zb - lz)
+                            : (ijk[0] * np + ijk[1]) * np + zb;
+                lb |= ijk[2] & 1;
+            }
+            s_lb[tid] = lb;
+        }
         if (staged) {
-            // box rows in 16-byte chunks: l4 threads per row (a power of two >=
-            // ceil(bz / 4)), 256 / l4 rows per pass, 8 passes' loads in flight
-            // before their LDS stores
+            // the box as nrows * q4 chunks of 16 bytes, chunk k = row k / q4,
+            // quarter k % q4: every thread's loads go in flight at once (at most
+            // kStageChunks each, enough for any box within kBoxCap), then the
+            // 8-byte LDS stores (pairs at or past S dropped)
             const int q4 = (bz + 3) >> 2;
-            const int l4 = q4 <= 4 ? 4 : (q4 <= 8 ? 8 : 16);
-            const int rpi = 256 / l4;
-            const int zq = tid & (l4 - 1), rl = tid / l4;
-            const int z0 = 4 * zq;
-            const int nrows = bx * by;
-            const int gz = lo[2] + z0;
-            const float inv_by = 1.0f / (float)by;
-            const int npass = (nrows + rpi - 1) / rpi;
-            for (int p0 = 0; p0 < npass; p0 += 8) {
-                uint4v v[8];
-                int dst[8];
+            const int nchunk = bx * by * q4;
+            const float inv_q4 = 1.0f / (float)q4, inv_by = 1.0f / (float)by;
+            auto stage = [&](int u0, auto nb) __attribute__((always_inline)) {
+                constexpr int NB = decltype(nb)::value;
+                uint4v v[NB];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int r = (p0 + u) * rpi + rl;
-                    const bool in = (p0 + u) < npass && r < nrows && zq < q4;
-                    // r / by for r < 2^16: the float estimate is within 0.5/by of an integer
-                    const int x = (int)(((float)r + 0.5f) * inv_by), y = r - x * by;
-                    const int gidx = in ? ((lo[0] + x) * np + (lo[1] + y)) * np + gz : 0;
+                for (int u = 0; u < NB; ++u) {
+                    const int k = (u0 + u) * 256 + tid;
+                    // k / q4 and r / by for k < 2^16: the float estimates are
+                    // within half a unit of the integer quotient
+                    const int r = (int)(((float)k + 0.5f) * inv_q4), zq = k - r * q4;
+                    const int xr = (int)(((float)r + 0.5f) * inv_by), yr = r - xr * by;
+                    const int gidx =
+                        k < nchunk ? ((lo[0] + xr) * np + (lo[1] + yr)) * np + lz + 4 * zq : 0;
                     v[u] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, gidx * 4, 0, 0);
-                    dst[u] = in ? r * S + z0 : -1;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (dst[u] >= 0) {
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (z0 + e < bz) s_box[dst[u] + e] = __uint_as_float(v[u][e]);
+                for (int u = 0; u < NB; ++u) {
+                    const int k = (u0 + u) * 256 + tid;
+                    if (k < nchunk) {
+                        const int r = (int)(((float)k + 0.5f) * inv_q4), z4 = 4 * (k - r * q4);
+                        float2 *d2 = reinterpret_cast<float2 *>(s_box + r * S + z4);
+                        d2[0] = make_float2(__uint_as_float(v[u][0]), __uint_as_float(v[u][1]));
+                        if (z4 + 2 < S)
+                            d2[1] = make_float2(__uint_as_float(v[u][2]), __uint_as_float(v[u][3]));
                     }
                 }
-            }
+            };
+            stage(0, std::integral_constant<int, kStageChunks>{});
+            for (int u0 = kStageChunks; u0 * 256 < nchunk; u0 += 4)
+                stage(u0, std::integral_constant<int, 4>{});
         }
         __syncthreads();
         // ---- taps: each wave its 64 points, PPW per step
+        const int plane = by * S;
 #pragma unroll 1
-        for (int it = 0; it < G; ++it) {
+        for (int it = 0; it < GP; ++it) {
             const int pl = wave * 64 + it * PPW + g;
-            const int4 pi = s_ijk[pl];
-            const float4 tt = s_t[pl];
-            float wa_c = 0.f, wb_c = 0.f, wc_c = 0.f;
-            if (c < W) {
-                wa_c = es_phi((float)c - tt.x, gv.beta_l2e, gv.c2);
-                wb_c = es_phi((float)c - tt.y, gv.beta_l2e, gv.c2);
-                wc_c = es_phi((float)c - tt.z, gv.beta_l2e, gv.c2);
-            }
-            s_w[wave][g][0][c] = wa_c;
-            s_w[wave][g][1][c] = wb_c;
-            wave_sync();
-            float wa[W], wb[W];
+            const int lv = s_lb[pl];
+            const float tx = s_t[0][pl], ty = s_t[1][pl], tz = s_t[2][pl];
+            const int sh = lv & 1;
+            const int base = lv >= 0 ? (lv & ~1) + 2 * c : 0;
+            float own_a[NS], own_b[NS];
 #pragma unroll
-            for (int a = 0; a < W; ++a) {
-                wa[a] = s_w[wave][g][0][a];
-                wb[a] = s_w[wave][g][1][a];
+            for (int s = 0; s < NS; ++s) {
+                own_a[s] = es_phi((float)(c + GP * s) - tx, gv.beta_l2e, gv.c2);
+                own_b[s] = es_phi((float)(c + GP * s) - ty, gv.beta_l2e, gv.c2);
             }
-            const bool valid = pi.w >= 0;
-            float acc = 0.f;
+            float wa[W], wb[W];
+            gather_weights<GP, W>(own_a, wa, lane);
+            gather_weights<GP, W>(own_b, wb, lane);
+            // elements 2c - sh and 2c + 1 - sh of the window (zero weight outside it)
+            const float k0 = (float)(2 * c - sh);
+            const float wc0 = es_phi(k0 - tz, gv.beta_l2e, gv.c2);
+            const float wc1 = es_phi(k0 + 1.0f - tz, gv.beta_l2e, gv.c2);
+            float r0 = 0.f, r1 = 0.f;
             if (staged) {
-                int lb = 0;
-                if (valid) lb = ((pi.x - lo[0]) * by + (pi.y - lo[1])) * S + (pi.z - lo[2]) + c;
                 switch (S) {
-#define MTIP_TAPS_CASE(SS) case SS: acc = taps_lds<W, G, SS>(s_box, lb, by, wa, wb); break;
-                    MTIP_STRIDE_CASES(MTIP_TAPS_CASE)
+#define MTIP_TAPS_CASE(SS) case SS: taps_pair_lds<W, SS>(s_box, base, plane, wa, wb, r0, r1); break;
+                    MTIP_PAIR_STRIDES(MTIP_TAPS_CASE)
 #undef MTIP_TAPS_CASE
                 default: break;
                 }
             } else {
-                const int voff = valid ? ((pi.x * np + pi.y) * np + pi.z + c) * 4 : 0;
-                acc = taps_direct<W>(rsrc, np, voff, wa, wb);
+                taps_pair_direct<W>(rsrc, np, base * 4, wa, wb, r0, r1);
             }
-            const float r = group_sum<G>(acc * wc_c);
-            if (c == 0 && pi.w != -1) {
-                float v = r;
-                int j = pi.w;
-                if (pi.w == -2) {
-                    const int64_t k = tb + pl;
-                    j = order ? order[k] : (int)k;
+            const float r = group_sum<GP>(fmaf(wc0, r0, wc1 * r1));
+            // the slot's value replaces its x offset (read above, by this wave only)
+            if (c == 0) s_t[0][pl] = r;
+        }
+        // ---- outputs: each thread its own slot (written by its own wave)
+        wave_sync();
+        {
+            const int pix_o = s_pix[tid];
+            if (pix_o >= 0) {
+                float v = s_t[0][tid];
+                if (s_lb[tid] < 0) {
                     v = __builtin_nanf("");
                     ++nbad;
                 }
-                orow[j] = v;
-                if (brow) brow[j] = f32_to_bf16_rne(v);
+                orow[pix_o] = v;
+                if (brow) brow[pix_o] = f32_to_bf16_rne(v);
                 sum += (double)v;
                 sumsq += (double)v * (double)v;
             }
-            wave_sync();
         }
         __syncthreads();
     }
@@ -757,8 +849,7 @@ void launch_slices(const GridView &gv, const double *quats, int64_t N_o, const d
                    int64_t N_p, const int32_t *order, int64_t n_order, double gscale, float *out,
                    int64_t ld, uint16_t *obf, double *rs, double *rq, int *bad, hipStream_t st)
 {
-    constexpr int G = W <= 4 ? 4 : (W <= 8 ? 8 : 16);
-    hipLaunchKernelGGL((k_slices_lds<W, G>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q,
+    hipLaunchKernelGGL((k_slices_lds<W>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q,
                        N_p, order, n_order, gscale, out, ld, obf, rs, rq, bad);
 }
 
