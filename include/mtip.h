@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MTIP_ABI_VERSION 1
+#define MTIP_ABI_VERSION 2
 
 enum {
     MTIP_OK = 0,
@@ -109,18 +109,20 @@ int mtip_plan_execute(mtip_plan *plan, const double *x, const double *y, const d
  * sums of s and s^2 per orientation (for the std scaling :245-248 and the
  * squared norms of the distance expansion).  bad_count as for execute.
  * pixel_order (nullable, device int32, n_order entries, n_order a multiple
- * of 256): the order in which pixels are visited, each run of 256 entries
- * one tile of neighbouring pixels (e.g. 16 x 16 detector blocks), entries -1
- * for padding; every pixel in [0, N_p) must appear exactly once.  Tiles of
- * neighbouring pixels let the kernel stage one small box of the fine grid
- * per tile in LDS.  NULL visits pixels in index order, 256 at a time.
+ * of tile_points): the order in which pixels are visited, each run of
+ * tile_points entries one tile of neighbouring pixels (16 x 16, 8 x 16 or
+ * 8 x 8 detector blocks for tile_points 256, 128, 64; 0 means 256), entries
+ * -1 for padding; every pixel in [0, N_p) must appear exactly once.  Tiles
+ * of neighbouring pixels let the kernel stage one small box of the fine grid
+ * per tile in LDS; smaller tiles keep the box of a large grid (M >= 121)
+ * within LDS.  NULL visits pixels in index order, tile_points at a time.
  * Replaces the batched loop of spinifel/sequential/orientation_matching.py:230-241
  * and gen_model_slices (spinifel/slicing.py:10-45).
  */
 int mtip_plan_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
-                     int64_t N_p, const int32_t *pixel_order, int64_t n_order, double scale,
-                     float *out, int64_t ld_out, uint16_t *out_bf16, double *row_sum,
-                     double *row_sumsq, int *bad_count, void *stream);
+                     int64_t N_p, const int32_t *pixel_order, int64_t n_order, int tile_points,
+                     double scale, float *out, int64_t ld_out, uint16_t *out_bf16,
+                     double *row_sum, double *row_sumsq, int *bad_count, void *stream);
 
 /*
  * Rotated, normalised q-points only: H/K/L (device FP64, N_o * N_p,
@@ -199,6 +201,18 @@ int mtip_col_mean(const float *in, int64_t rows, int64_t cols, int64_t ld, doubl
 size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision);
 
 /*
+ * host: orientation rows per GEMM tile for `precision`: 0 restores the
+ * built-in rule (256 for BF16 with N_o, N_i >= 1024, else 128), 128 or 256
+ * force a tile.  Process-wide; set it before mtip_match_partial and keep it
+ * until the matching mtip_topk_merge.  No counterpart in the reference (the
+ * reference has one GEMM path); the project's autotuner sets it from timings.
+ */
+int mtip_set_gemm_tile(int precision, int rows);
+
+/* host: the tile rows mtip_match_partial will use for these sizes (>0), or an error code. */
+int mtip_gemm_tile(int precision, int64_t N_o, int64_t N_i);
+
+/*
  * Per image i the K orientations with the smallest
  *   v(n, i) = alpha^2 ||s_n||^2 - 2 alpha s_n . d_i
  * (= squared distance minus the image's own ||d_i||^2, which does not move
@@ -275,6 +289,64 @@ int mtip_euclidean_distance(const float *ref, const float *data, int64_t numRef,
  */
 int mtip_heap_sort(const float *euDist, int64_t numData, int64_t numRef, int totalNN,
                    int32_t *idx, void *stream);
+
+/* ------------------------------------------------------------- orientation hierarchy */
+
+/*
+ * Batched refinement over a hierarchy of orientation grids (no counterpart
+ * in the reference, whose matching is one flat grid: the batched loop of
+ * spinifel/sequential/orientation_matching.py:230-256 over all N_o
+ * orientations).  After a dense top-T against a coarse grid, each image
+ * refines among the children of its T winners on the next grid.  An entry
+ * is one (image i, winner t) pair, e = i * n_win + t.
+ *
+ * host: bytes of the grouping workspace for N_e entries and n_groups coarse nodes.
+ */
+size_t mtip_group_workspace_size(int64_t N_e, int64_t n_groups);
+
+/*
+ * Groups entries by winner: winners device int32, winner t of image i at
+ * winners[i * win_stride + t] (t < n_win <= win_stride; e.g. the first
+ * n_win columns of a (N_i, K) top-K index), in [0, n_groups) (others are
+ * counted as bad and left out).  Fills the workspace with group sizes and
+ * offsets, the entries of each group and a task list of (group, up to 32
+ * entries), all on the device: no host synchronisation and no memset, so
+ * the call can be captured in a graph.
+ */
+int mtip_group_by_winner(const int32_t *winners, int64_t win_stride, int n_win, int64_t N_i,
+                         int64_t n_groups, void *workspace, size_t ws_bytes, void *stream);
+
+/*
+ * Per entry, the K children of its winner with the smallest
+ *   v = ||s_c||^2 - 2 s_c . d_i
+ * on bf16 MFMA with FP32 accumulation (the centred, scaled operands of
+ * mtip_match_topk with alpha = 1).  child_off (n_groups + 1) / child_idx:
+ * device int32 CSR of each coarse node's children (row indices of
+ * `slices`); slices (N_fine, ld_s) bf16, slice_sumsq FP64 (N_fine);
+ * images (N_i, ld_d) bf16; kd: pixels summed, a multiple of 64 within both
+ * pitches (columns past N_p zero).  A persistent kernel walks the task list
+ * the previous mtip_group_by_winner (same n_win, N_i, n_groups) left in
+ * `workspace`.  Outputs top_idx int32 / top_val float32 (N_i * n_win, K),
+ * row e for entry e, ascending, lowest index first on ties, -1 / +inf past
+ * the group's children; entries left out are not written.  K in {1, 2, 4, 8}.
+ */
+int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, int64_t n_groups,
+                    const int32_t *child_off, const int32_t *child_idx, const uint16_t *slices,
+                    int64_t ld_s, const double *slice_sumsq, const uint16_t *images, int64_t ld_d,
+                    int64_t kd, int K, int32_t *top_idx, float *top_val, int device, void *stream);
+
+/* Number of entries the last mtip_group_by_winner left out (device int32 copy). */
+int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_i, int64_t n_groups,
+                         int32_t *out, void *stream);
+
+/*
+ * Per image, the K_out smallest distinct (value, index) candidates among
+ * its n_in (in_idx / in_val, (N_i, n_in), index -1 = none): merges an
+ * image's entries after mtip_group_topk.  Outputs (N_i, K_out) ascending,
+ * lowest index first on ties, -1 / +inf when fewer.  K_out in {1, 2, 4, 8}.
+ */
+int mtip_merge_distinct(const int32_t *in_idx, const float *in_val, int64_t N_i, int n_in,
+                        int K_out, int32_t *out_idx, float *out_val, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,410 @@
+// Batched matching against a hierarchy of orientation grids (gfx950).
+//
+// After a dense top-K against the coarse grid, every image refines around
+// its winner on the next grid: the candidates are the winner's children (the
+// finer nodes that have the winner among their nearest coarser nodes).
+// Images are grouped by winner on the device (count, scan, scatter, task
+// list), so the images of one group share one read of the children's slices;
+// a persistent kernel then walks the (group, 32-image chunk) tasks.  An
+// image may follow several winners (a beam of T): grouping then works on
+// entries (image, t), each entry gets its own top-K, and a merge keeps each
+// image's best distinct candidates.  No
+// step needs the host, so a whole multi-level pass can be captured in a
+// hipGraph and replayed.
+//
+// Grouped top-K block: 32 children (MFMA A rows) x 32 images (B rows) on a
+// v_mfma_f32_32x32x16_bf16; the four waves of a workgroup take interleaved
+// 64-pixel chunks and their partial products are summed in LDS.  Each
+// lane streams its own operand rows with 16-byte loads (rows are gathered,
+// so there is nothing to share through LDS).  Within each 64-element chunk
+// of pixels, lane half h feeds elements [32 h, 32 h + 32) in four MFMA
+// steps, for both operands alike, so every pixel meets itself.
+#include "mtip_internal.h"
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTaskImages = 32;     // images per task (one MFMA block of columns)
+constexpr int kChunk = 32;          // children per MFMA block of rows
+constexpr uint64_t KEY_NONE = ~0ull;
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Group sizes over entries e = i * n_win + t (image i, its t-th winner):
+// counts[g] += 1 for every entry whose winner is g.
+__global__ void k_count(const int32_t *__restrict__ win, int64_t win_stride, int64_t n_win,
+                        int64_t N_e, int64_t n_groups, int32_t *__restrict__ counts,
+                        int *__restrict__ bad)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N_e) return;
+    const int32_t g = win[(e / n_win) * win_stride + e % n_win];
+    if (g >= 0 && g < n_groups) atomicAdd(&counts[g], 1);
+    else atomicAdd(bad, 1);
+}
+
+// One workgroup: exclusive scans of the group sizes (image offsets) and of
+// the per-group task counts; cursor = image offsets (for the scatter);
+// n_tasks[0] = number of tasks.
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ counts, int64_t n_groups,
+                                               int32_t *__restrict__ img_off,
+                                               int32_t *__restrict__ cursor,
+                                               int32_t *__restrict__ task_off,
+                                               int32_t *__restrict__ n_tasks)
+{
+    __shared__ int32_t s_a[1024], s_b[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (n_groups + 1023) / 1024;
+    const int64_t g0 = tid * per, g1 = min(n_groups, g0 + per);
+    int32_t a = 0, b = 0;
+    for (int64_t g = g0; g < g1; ++g) {
+        a += counts[g];
+        b += (counts[g] + kTaskImages - 1) / kTaskImages;
+    }
+    s_a[tid] = a;
+    s_b[tid] = b;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 partial sums
+    for (int d = 1; d < 1024; d <<= 1) {
+        int32_t va = tid >= d ? s_a[tid - d] : 0, vb = tid >= d ? s_b[tid - d] : 0;
+        __syncthreads();
+        s_a[tid] += va;
+        s_b[tid] += vb;
+        __syncthreads();
+    }
+    int32_t ra = s_a[tid] - a, rb = s_b[tid] - b;     // exclusive
+    for (int64_t g = g0; g < g1; ++g) {
+        img_off[g] = ra;
+        cursor[g] = ra;
+        task_off[g] = rb;
+        ra += counts[g];
+        rb += (counts[g] + kTaskImages - 1) / kTaskImages;
+    }
+    if (tid == 1023) {
+        img_off[n_groups] = s_a[1023];
+        n_tasks[0] = s_b[1023];
+    }
+}
+
+// perm[pos] = entry, the entries of a group contiguous (order within a group
+// is arbitrary; every per-entry result is independent of it).
+__global__ void k_scatter(const int32_t *__restrict__ win, int64_t win_stride, int64_t n_win,
+                          int64_t N_e, int64_t n_groups, int32_t *__restrict__ cursor,
+                          int32_t *__restrict__ perm)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N_e) return;
+    const int32_t g = win[(e / n_win) * win_stride + e % n_win];
+    if (g >= 0 && g < n_groups) perm[atomicAdd(&cursor[g], 1)] = (int32_t)e;
+}
+
+// tasks[task_off[g] + t] = (g, first image position, end position)
+__global__ void k_tasks(const int32_t *__restrict__ counts, const int32_t *__restrict__ img_off,
+                        const int32_t *__restrict__ task_off, int64_t n_groups,
+                        int4 *__restrict__ tasks)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const int32_t n = counts[g], b = img_off[g];
+    int32_t t = task_off[g];
+    for (int32_t s = 0; s < n; s += kTaskImages, ++t)
+        tasks[t] = make_int4((int32_t)g, b + s, b + min(n, s + kTaskImages), 0);
+}
+
+__device__ __forceinline__ uint4 ld16(const uint16_t *row, int64_t e)
+{
+    return row ? *(const uint4 *)(row + e) : make_uint4(0, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
+{
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// Per task: the K children of the group with the smallest
+//   v = ||s_c||^2 - 2 s_c . d_i
+// for each of up to 32 images (the centred, scaled operands of the dense
+// path, so v ranks like the squared distance).  kd: pixels summed, a
+// multiple of 64 within both row pitches; columns past N_p are zero.
+template <int K>
+__global__ __launch_bounds__(256) void k_group_topk(
+    const int4 *__restrict__ tasks, const int32_t *__restrict__ n_tasks,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ child_off,
+    const int32_t *__restrict__ child_idx, const uint16_t *__restrict__ S, int64_t lds,
+    const double *__restrict__ s_sumsq, const uint16_t *__restrict__ D, int64_t ldd, int64_t kd,
+    int n_win, int32_t *__restrict__ out_idx, float *__restrict__ out_val)
+{
+    __shared__ float s_part[3][64][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    // wave w takes the 64-element chunks w, w + 4, w + 8, ... of [0, kd)
+    const int64_t k0 = 64 * wave;
+    const int nt = n_tasks[0];
+
+    for (int task = blockIdx.x; task < nt; task += gridDim.x) {
+        const int4 tk = tasks[task];
+        const int g = tk.x, p0 = tk.y, p1 = tk.z;
+        const int c_begin = child_off[g], c_end = child_off[g + 1];
+        // this lane's image row (B operand column l32): entry e is image e / n_win
+        const uint16_t *drow =
+            (p0 + l32 < p1) ? D + (int64_t)(perm[p0 + l32] / n_win) * ldd : nullptr;
+        uint64_t top[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r) top[r] = KEY_NONE;
+
+        for (int c0 = c_begin; c0 < c_end; c0 += kChunk) {
+            const int crow = c0 + l32;
+            const int child = crow < c_end ? child_idx[crow] : -1;
+            const uint16_t *srow = child >= 0 ? S + (int64_t)child * lds : nullptr;
+            floatx16 acc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+            // 64-element chunks: loads of the next chunk in flight during this one's MFMAs
+            uint4 a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                a[s] = k0 < kd ? ld16(srow, k0 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                b[s] = k0 < kd ? ld16(drow, k0 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+            }
+            for (int64_t k = k0; k < kd; k += 256) {
+                uint4 an[4], bn[4];
+                const bool more = k + 256 < kd;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    an[s] = more ? ld16(srow, k + 256 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                    bn[s] = more ? ld16(drow, k + 256 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b[s]),
+                                                                  acc, 0, 0, 0);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    a[s] = an[s];
+                    b[s] = bn[s];
+                }
+            }
+            // sum the four waves' partial products in wave 0
+            if (wave > 0)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) s_part[wave - 1][lane][v] = acc[v];
+            __syncthreads();
+            if (wave == 0) {
+#pragma unroll
+                for (int w = 0; w < 3; ++w)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) acc[v] += s_part[w][lane][v];
+                // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int r = c0 + 8 * (v >> 2) + 4 * half + (v & 3);
+                    if (r < c_end) {
+                        const int cid = child_idx[r];
+                        const float val = fmaf(-2.0f, acc[v], (float)s_sumsq[cid]);
+                        mtip::topk_insert<K>(top, mtip::make_key(val, (uint32_t)cid));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (wave == 0) {
+            uint64_t other[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[r], 32);
+            mtip::topk_merge<K>(top, other);
+            if (half == 0 && p0 + l32 < p1) {
+                const int64_t img = perm[p0 + l32];     // the entry (its own output row)
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const bool ok = top[r] != KEY_NONE;
+                    out_idx[img * K + r] = ok ? (int32_t)(uint32_t)top[r] : -1;
+                    out_val[img * K + r] =
+                        ok ? mtip::ordered_to_float((uint32_t)(top[r] >> 32)) : __builtin_inff();
+                }
+            }
+        }
+    }
+}
+
+// Per image: the K_out smallest distinct keys among its n_in candidate
+// (index, value) pairs (T entries x K each, index -1 = none).
+template <int KO>
+__global__ void k_merge_distinct(const int32_t *__restrict__ in_idx,
+                                 const float *__restrict__ in_val, int64_t N_i, int n_in,
+                                 int32_t *__restrict__ out_idx, float *__restrict__ out_val)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N_i) return;
+    uint64_t top[KO];
+#pragma unroll
+    for (int r = 0; r < KO; ++r) top[r] = KEY_NONE;
+    for (int j = 0; j < n_in; ++j) {
+        const int32_t id = in_idx[i * n_in + j];
+        if (id < 0) continue;
+        const uint64_t key = mtip::make_key(in_val[i * n_in + j], (uint32_t)id);
+        bool dup = false;
+#pragma unroll
+        for (int r = 0; r < KO; ++r) dup |= (top[r] == key);
+        if (!dup) mtip::topk_insert<KO>(top, key);
+    }
+#pragma unroll
+    for (int r = 0; r < KO; ++r) {
+        const bool ok = top[r] != KEY_NONE;
+        out_idx[i * KO + r] = ok ? (int32_t)(uint32_t)top[r] : -1;
+        out_val[i * KO + r] = ok ? mtip::ordered_to_float((uint32_t)(top[r] >> 32)) : __builtin_inff();
+    }
+}
+
+int n_blocks_persistent(int device)
+{
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return 4 * cus;
+}
+
+}  // namespace
+
+extern "C" size_t mtip_group_workspace_size(int64_t N_e, int64_t n_groups)
+{
+    // counts, img_off (+1), cursor, task_off, n_tasks, bad | perm | tasks (int4)
+    const int64_t ints = 4 * (n_groups + 1) + 2 + N_e;
+    const int64_t max_tasks = N_e / kTaskImages + n_groups + 1;
+    return (size_t)(((ints * 4 + 15) / 16) * 16 + max_tasks * 16);
+}
+
+namespace {
+struct GroupWs {
+    int32_t *counts, *img_off, *cursor, *task_off, *n_tasks, *bad, *perm;
+    int4 *tasks;
+};
+GroupWs carve(void *ws, int64_t N_i, int64_t n_groups)
+{
+    GroupWs w;
+    int32_t *p = (int32_t *)ws;
+    w.counts = p;
+    p += n_groups + 1;
+    w.img_off = p;
+    p += n_groups + 1;
+    w.cursor = p;
+    p += n_groups + 1;
+    w.task_off = p;
+    p += n_groups + 1;
+    w.n_tasks = p++;
+    w.bad = p++;
+    w.perm = p;
+    p += N_i;
+    const int64_t ints = 4 * (n_groups + 1) + 2 + N_i;
+    w.tasks = (int4 *)((uint8_t *)ws + ((ints * 4 + 15) / 16) * 16);
+    return w;
+}
+}  // namespace
+
+extern "C" int mtip_group_by_winner(const int32_t *winners, int64_t win_stride, int n_win,
+                                    int64_t N_i, int64_t n_groups, void *workspace,
+                                    size_t ws_bytes, void *stream)
+{
+    MTIP_REQUIRE(N_i >= 0 && n_groups > 0 && n_win >= 1 && win_stride >= n_win, MTIP_ERR_ARG,
+                 "mtip_group_by_winner: bad sizes");
+    const int64_t N_e = N_i * n_win;
+    MTIP_REQUIRE(N_e <= 0x7fffffff && n_groups <= 0x7fffffff, MTIP_ERR_ARG,
+                 "mtip_group_by_winner: sizes exceed int32");
+    MTIP_REQUIRE(workspace, MTIP_ERR_ARG, "mtip_group_by_winner: null workspace");
+    MTIP_REQUIRE(ws_bytes >= mtip_group_workspace_size(N_e, n_groups), MTIP_ERR_ARG,
+                 "mtip_group_by_winner: workspace too small");
+    MTIP_REQUIRE(N_i == 0 || winners, MTIP_ERR_ARG, "mtip_group_by_winner: null winners");
+    hipStream_t st = (hipStream_t)stream;
+    GroupWs w = carve(workspace, N_e, n_groups);
+    // counts, n_tasks and bad start at zero (a kernel, not a memset, so a
+    // captured graph can replay it)
+    const int64_t nz = 4 * (n_groups + 1) + 2;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)mtip::ceil_div(nz, 256)), dim3(256), 0, st,
+                       w.counts, nz, 0);
+    if (N_e > 0)
+        hipLaunchKernelGGL(k_count, dim3((unsigned)mtip::ceil_div(N_e, 256)), dim3(256), 0, st,
+                           winners, win_stride, (int64_t)n_win, N_e, n_groups, w.counts, w.bad);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, w.counts, n_groups, w.img_off,
+                       w.cursor, w.task_off, w.n_tasks);
+    if (N_e > 0)
+        hipLaunchKernelGGL(k_scatter, dim3((unsigned)mtip::ceil_div(N_e, 256)), dim3(256), 0, st,
+                           winners, win_stride, (int64_t)n_win, N_e, n_groups, w.cursor, w.perm);
+    hipLaunchKernelGGL(k_tasks, dim3((unsigned)mtip::ceil_div(n_groups, 256)), dim3(256), 0, st,
+                       w.counts, w.img_off, w.task_off, n_groups, w.tasks);
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, int64_t n_groups,
+                               const int32_t *child_off, const int32_t *child_idx,
+                               const uint16_t *slices, int64_t ld_s, const double *slice_sumsq,
+                               const uint16_t *images, int64_t ld_d, int64_t kd, int K,
+                               int32_t *top_idx, float *top_val, int device, void *stream)
+{
+    MTIP_REQUIRE(N_i >= 0 && n_groups > 0 && n_win >= 1, MTIP_ERR_ARG,
+                 "mtip_group_topk: bad sizes");
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG,
+                 "mtip_group_topk: K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(kd > 0 && kd % 64 == 0, MTIP_ERR_ARG,
+                 "mtip_group_topk: kd must be a positive multiple of 64");
+    MTIP_REQUIRE(ld_s >= kd && ld_d >= kd && ld_s % 8 == 0 && ld_d % 8 == 0, MTIP_ERR_ARG,
+                 "mtip_group_topk: row pitches must be >= kd and multiples of 8");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(workspace && child_off && child_idx && slices && slice_sumsq && images &&
+                     top_idx && top_val,
+                 MTIP_ERR_ARG, "mtip_group_topk: null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
+    const int nb = n_blocks_persistent(device);
+#define MTIP_GROUP_CASE(KK)                                                                    \
+    hipLaunchKernelGGL((k_group_topk<KK>), dim3(nb), dim3(256), 0, st, w.tasks, w.n_tasks,     \
+                       w.perm, child_off, child_idx, slices, ld_s, slice_sumsq, images, ld_d,  \
+                       kd, n_win, top_idx, top_val)
+    switch (K) {
+    case 1: MTIP_GROUP_CASE(1); break;
+    case 2: MTIP_GROUP_CASE(2); break;
+    case 4: MTIP_GROUP_CASE(4); break;
+    default: MTIP_GROUP_CASE(8); break;
+    }
+#undef MTIP_GROUP_CASE
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+extern "C" int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_i,
+                                    int64_t n_groups, int32_t *out, void *stream)
+{
+    MTIP_REQUIRE(workspace && out && n_win >= 1, MTIP_ERR_ARG, "mtip_group_bad_count: bad args");
+    GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
+    MTIP_HIP_CHECK(hipMemcpyAsync(out, w.bad, sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                  (hipStream_t)stream));
+    return MTIP_OK;
+}
+
+extern "C" int mtip_merge_distinct(const int32_t *in_idx, const float *in_val, int64_t N_i,
+                                   int n_in, int K_out, int32_t *out_idx, float *out_val,
+                                   void *stream)
+{
+    MTIP_REQUIRE(N_i >= 0 && n_in >= 1, MTIP_ERR_ARG, "mtip_merge_distinct: bad sizes");
+    MTIP_REQUIRE(K_out == 1 || K_out == 2 || K_out == 4 || K_out == 8, MTIP_ERR_ARG,
+                 "mtip_merge_distinct: K_out must be 1, 2, 4 or 8");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(in_idx && in_val && out_idx && out_val, MTIP_ERR_ARG,
+                 "mtip_merge_distinct: null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((unsigned)mtip::ceil_div(N_i, 256));
+    switch (K_out) {
+    case 1: hipLaunchKernelGGL(k_merge_distinct<1>, grid, dim3(256), 0, st, in_idx, in_val, N_i, n_in, out_idx, out_val); break;
+    case 2: hipLaunchKernelGGL(k_merge_distinct<2>, grid, dim3(256), 0, st, in_idx, in_val, N_i, n_in, out_idx, out_val); break;
+    case 4: hipLaunchKernelGGL(k_merge_distinct<4>, grid, dim3(256), 0, st, in_idx, in_val, N_i, n_in, out_idx, out_val); break;
+    default: hipLaunchKernelGGL(k_merge_distinct<8>, grid, dim3(256), 0, st, in_idx, in_val, N_i, n_in, out_idx, out_val); break;
+    }
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}

@@ -18,6 +18,8 @@
 // order the hardware sums in.
 #include <math.h>
 
+#include <atomic>
+
 #include "mtip_internal.h"
 
 namespace {
@@ -848,10 +850,18 @@ __global__ __launch_bounds__(256) void k_col_finish(const double *__restrict__ w
 }
 
 
-// Orientation rows per top-K partial tile: 256 (big kernel) for large
-// problems, else 128.  The partial buffer and the merge use the same value.
+// Per-precision tile override (0 = rule below), set by mtip_set_gemm_tile.
+std::atomic<int> g_tile_override[3] = {{0}, {0}, {0}};
+
+// Orientation rows per top-K partial tile: 256 (big kernel) for large BF16
+// problems, else 128, unless overridden.  The partial buffer and the merge
+// use the same value.
 int tile_rows(int precision, int64_t N_o, int64_t N_i)
 {
+    if (precision >= 0 && precision < 3) {
+        const int o = g_tile_override[precision].load(std::memory_order_relaxed);
+        if (o) return o;
+    }
     return (precision == MTIP_BF16 && N_o >= 1024 && N_i >= 1024) ? BM2 : BM;
 }
 
@@ -927,6 +937,23 @@ int check_match_args(const char *fn, int precision, const void *slices, int64_t 
 }
 
 }  // namespace
+
+extern "C" int mtip_set_gemm_tile(int precision, int rows)
+{
+    MTIP_REQUIRE(precision == MTIP_FP32 || precision == MTIP_BF16 || precision == MTIP_FP8,
+                 MTIP_ERR_ARG, "mtip_set_gemm_tile: unknown precision");
+    MTIP_REQUIRE(rows == 0 || rows == BM || rows == BM2, MTIP_ERR_ARG,
+                 "mtip_set_gemm_tile: rows must be 0, 128 or 256");
+    g_tile_override[precision].store(rows, std::memory_order_relaxed);
+    return MTIP_OK;
+}
+
+extern "C" int mtip_gemm_tile(int precision, int64_t N_o, int64_t N_i)
+{
+    MTIP_REQUIRE(precision == MTIP_FP32 || precision == MTIP_BF16 || precision == MTIP_FP8,
+                 MTIP_ERR_ARG, "mtip_gemm_tile: unknown precision");
+    return tile_rows(precision, N_o, N_i);
+}
 
 extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_s,
                                   const float *slice_scale, const double *slice_sumsq, int64_t N_o,

@@ -427,8 +427,9 @@ __global__ __launch_bounds__(256) void k_interp(GridView gv, const double *__res
 
 // ------------------------------------------------------------ LDS-staged slicing
 //
-// One workgroup per orientation walks the detector in tiles of kTile pixels
-// (16 x 16 blocks when the caller passes a tile order).  A tile's points lie
+// One workgroup per orientation walks the detector in tiles of TP = 256, 128
+// or 64 pixels (16 x 16, 8 x 16 or 8 x 8 blocks when the caller passes a tile
+// order; the smaller tiles keep the boxes of large grids within LDS).  A tile's points lie
 // on a small patch of the rotated Ewald sphere, so the union of their w^3
 // windows fits a small box of the fine grid (about 8k floats at M = 81, 0.9
 // Nyquist): the workgroup stages that box in LDS with 16-byte loads, then
@@ -442,7 +443,6 @@ __global__ __launch_bounds__(256) void k_interp(GridView gv, const double *__res
 // 47 KiB of box: with the point arrays a workgroup takes under 53.4 KiB of
 // LDS, so three fit the 160 KiB of a CU.
 constexpr int kBoxCap = 12032;
-constexpr int kTile = 256;        // pixels per workgroup tile
 // 16-byte staging chunks per thread loaded in one batch: a typical box at
 // M = 81 needs about 8, the largest within kBoxCap up to 15 (the rest follow
 // in batches of 4; a larger first batch costs registers and occupancy)
@@ -603,7 +603,7 @@ __device__ __forceinline__ int pair_stride(int bz)
          : bz <= 56 ? 56 : bz <= 64 ? 64 : 0;
 }
 
-template <int W>
+template <int W, int TP>
 __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double *__restrict__ quats,
                                                     const double *__restrict__ q, int64_t N_p,
                                                     const int32_t *__restrict__ order,
@@ -615,12 +615,14 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
 {
     using PT = PairTaps<W>;
     constexpr int GP = PT::GP, NS = PT::NS, PPW = 64 / GP;
+    constexpr int PW = TP / 4;                                  // points per wave
+    constexpr int STEPS = PW >= PPW ? PW / PPW : 1;
     __shared__ __align__(16) float s_box[kBoxCap];
     // per point: pair-base offset (box-local when staged, padded grid otherwise)
     // with the window parity in bit 0, or -1 for a point outside the domain
-    __shared__ int s_lb[kTile];
-    __shared__ int s_pix[kTile];       // output pixel, -1 for an empty slot
-    __shared__ float s_t[3][kTile];    // fractional window offsets
+    __shared__ int s_lb[TP];
+    __shared__ int s_pix[TP];          // output pixel, -1 for an empty slot
+    __shared__ float s_t[3][TP];       // fractional window offsets
     __shared__ int s_red_i[4][6];
     __shared__ double s_red[8];
 
@@ -644,11 +646,13 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
     uint16_t *brow = out_bf16 ? out_bf16 + o * ld_out : nullptr;
     double sum = 0.0, sumsq = 0.0;
     int nbad = 0;
-    const int64_t n_slots = order ? n_order : mtip::ceil_div(N_p, kTile) * kTile;
+    const int64_t n_slots = order ? n_order : mtip::ceil_div(N_p, TP) * TP;
+    const bool has_pt = tid < TP;      // threads holding a point of the tile
 
     // pixel index and q-vector of this thread's point, prefetched one tile ahead
     // (and the pixel index two tiles ahead) so their latency hides behind taps
     auto pixel_at = [&](int64_t k) -> int {
+        if (!has_pt) return -1;
         if (order) return (k < n_order) ? order[k] : -1;
         return (k < N_p) ? (int)k : -1;
     };
@@ -660,9 +664,9 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
         qy_n = q[N_p + pp];
         qz_n = q[2 * N_p + pp];
     }
-    int pix_nn = pixel_at(kTile + tid);
+    int pix_nn = pixel_at(TP + tid);
 
-    for (int64_t tb = 0; tb < n_slots; tb += kTile) {
+    for (int64_t tb = 0; tb < n_slots; tb += TP) {
         // ---- geometry of this thread's point
         const int pix = pix_n;
         const double x = qx_n, y = qy_n, z = qz_n;
@@ -673,7 +677,7 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
             qy_n = q[N_p + pp];
             qz_n = q[2 * N_p + pp];
         }
-        pix_nn = pixel_at(tb + 2 * kTile + tid);
+        pix_nn = pixel_at(tb + 2 * TP + tid);
         bool ok = false;
         int ijk[3] = {0, 0, 0};
         float t[3] = {0.f, 0.f, 0.f};
@@ -683,10 +687,12 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
             double X2 = (s_ri[6] * x + s_ri[7] * y + s_ri[8] * z) * gscale;
             ok = point_corner<W>(gv, X0, X1, X2, ijk, t);
         }
-        s_pix[tid] = pix;
-        s_t[0][tid] = t[0];
-        s_t[1][tid] = t[1];
-        s_t[2][tid] = t[2];
+        if (has_pt) {
+            s_pix[tid] = pix;
+            s_t[0][tid] = t[0];
+            s_t[1][tid] = t[1];
+            s_t[2][tid] = t[2];
+        }
         // ---- box of the tile's windows (block min / max)
         {
             int v[6] = {ok ? ijk[0] : 0x7fffffff, ok ? ijk[1] : 0x7fffffff, ok ? ijk[2] : 0x7fffffff,
@@ -719,7 +725,7 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
                             : (ijk[0] * np + ijk[1]) * np + zb;
                 lb |= ijk[2] & 1;
             }
-            s_lb[tid] = lb;
+            if (has_pt) s_lb[tid] = lb;
         }
         if (staged) {
             // the box as nrows * q4 chunks of 16 bytes, chunk k = row k / q4,
@@ -760,11 +766,12 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
                 stage(u0, std::integral_constant<int, 4>{});
         }
         __syncthreads();
-        // ---- taps: each wave its 64 points, PPW per step
+        // ---- taps: each wave its PW points, PPW per step
         const int plane = by * S;
 #pragma unroll 1
-        for (int it = 0; it < GP; ++it) {
-            const int pl = wave * 64 + it * PPW + g;
+        for (int it = 0; it < STEPS; ++it) {
+            const bool slot_ok = PW >= PPW || g < PW;
+            const int pl = slot_ok ? wave * PW + it * PPW + g : 0;
             const int lv = s_lb[pl];
             const float tx = s_t[0][pl], ty = s_t[1][pl], tz = s_t[2][pl];
             const int sh = lv & 1;
@@ -795,11 +802,12 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
             }
             const float r = group_sum<GP>(fmaf(wc0, r0, wc1 * r1));
             // the slot's value replaces its x offset (read above, by this wave only)
-            if (c == 0) s_t[0][pl] = r;
+            if (c == 0 && slot_ok) s_t[0][pl] = r;
         }
-        // ---- outputs: each thread its own slot (written by its own wave)
-        wave_sync();
-        {
+        // ---- outputs: each thread its own slot (written by its own wave when TP = 256)
+        if constexpr (TP == 256) wave_sync();
+        else __syncthreads();
+        if (has_pt) {
             const int pix_o = s_pix[tid];
             if (pix_o >= 0) {
                 float v = s_t[0][tid];
@@ -846,11 +854,17 @@ __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ qu
 
 template <int W>
 void launch_slices(const GridView &gv, const double *quats, int64_t N_o, const double *q,
-                   int64_t N_p, const int32_t *order, int64_t n_order, double gscale, float *out,
-                   int64_t ld, uint16_t *obf, double *rs, double *rq, int *bad, hipStream_t st)
+                   int64_t N_p, const int32_t *order, int64_t n_order, int tile, double gscale,
+                   float *out, int64_t ld, uint16_t *obf, double *rs, double *rq, int *bad,
+                   hipStream_t st)
 {
-    hipLaunchKernelGGL((k_slices_lds<W>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q,
-                       N_p, order, n_order, gscale, out, ld, obf, rs, rq, bad);
+#define MTIP_SLICES_TP(T)                                                                     \
+    hipLaunchKernelGGL((k_slices_lds<W, T>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q, \
+                       N_p, order, n_order, gscale, out, ld, obf, rs, rq, bad)
+    if (tile == 64) MTIP_SLICES_TP(64);
+    else if (tile == 128) MTIP_SLICES_TP(128);
+    else MTIP_SLICES_TP(256);
+#undef MTIP_SLICES_TP
 }
 
 template <int W>
@@ -1048,7 +1062,7 @@ extern "C" int mtip_plan_execute(mtip_plan *p, const double *x, const double *y,
 
 extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, const double *q,
                                 int64_t N_p, const int32_t *pixel_order, int64_t n_order,
-                                double scale, float *out, int64_t ld_out,
+                                int tile_points, double scale, float *out, int64_t ld_out,
                                 uint16_t *out_bf16, double *row_sum, double *row_sumsq,
                                 int *bad_count, void *stream)
 {
@@ -1061,14 +1075,18 @@ extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, 
     if (N_o == 0 || N_p == 0) return MTIP_OK;
     MTIP_REQUIRE(quats && q && out, MTIP_ERR_ARG, "mtip_plan_slices: null pointer");
     MTIP_REQUIRE(N_p <= 0x7fffffff, MTIP_ERR_ARG, "mtip_plan_slices: N_p too large");
-    MTIP_REQUIRE(!pixel_order || (n_order > 0 && n_order % 256 == 0), MTIP_ERR_ARG,
-                 "mtip_plan_slices: n_order must be a positive multiple of 256");
+    if (tile_points == 0) tile_points = 256;
+    MTIP_REQUIRE(tile_points == 64 || tile_points == 128 || tile_points == 256, MTIP_ERR_ARG,
+                 "mtip_plan_slices: tile_points must be 0, 64, 128 or 256");
+    MTIP_REQUIRE(!pixel_order || (n_order > 0 && n_order % tile_points == 0), MTIP_ERR_ARG,
+                 "mtip_plan_slices: n_order must be a positive multiple of tile_points");
     DeviceGuard dg(p->device);
     GridView gv = view_of(p);
     // grid units per q unit: (pi / (E os)) * n / (2 pi)
     double gscale = scale * (double)p->n / (2.0 * M_PI);
-    MTIP_W_DISPATCH(p->w, launch_slices, gv, quats, N_o, q, N_p, pixel_order, n_order, gscale,
-                    out, ld_out, out_bf16, row_sum, row_sumsq, bad_count, (hipStream_t)stream);
+    MTIP_W_DISPATCH(p->w, launch_slices, gv, quats, N_o, q, N_p, pixel_order, n_order,
+                    tile_points, gscale, out, ld_out, out_bf16, row_sum, row_sumsq, bad_count,
+                    (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }

@@ -17,13 +17,14 @@ Pipeline of one ``match`` (all on one stream, inputs resident in HBM):
 """
 import ctypes
 from dataclasses import dataclass
+from typing import NamedTuple
 
 import numpy as np
 import torch
 
 from . import _lib
 
-_PAD = 16      # pixel padding multiple for the GEMM operands
+_PAD = 64     # row pitch multiple (elements) of every resident operand
 
 
 def _ptr(t):
@@ -91,11 +92,22 @@ def _span(timers, name):
     return timers.span(name) if timers is not None else _NoSpan()
 
 
+class Tiling(NamedTuple):
+    """Pixel visiting order for the slicing kernel: runs of `points` entries,
+    each one tile of neighbouring detector pixels, -1 padded."""
+    order: np.ndarray
+    points: int
+    shape: tuple
+
+
+TILE_SHAPES = {256: (16, 16), 128: (8, 16), 64: (8, 8)}
+
+
 def tile_order(det_shape, tile=(16, 16)):
-    """Pixel visiting order in runs of 256: 16 x 16 blocks of each panel, -1 padded.
+    """Pixel visiting order in runs of ty * tx: tile blocks of each panel, -1 padded.
 
     det_shape (..., rows, cols) with leading dims as panels; returns an int32
-    numpy array whose length is a multiple of 256, or None for a 1-D detector.
+    numpy array whose length is a multiple of ty * tx, or None for a 1-D detector.
     """
     det_shape = tuple(int(d) for d in det_shape)
     if len(det_shape) < 2:
@@ -112,11 +124,16 @@ def tile_order(det_shape, tile=(16, 16)):
                 run = np.full(ty * tx, -1, dtype=np.int64)
                 run[:blk.size] = blk
                 runs.append(run)
-    out = np.concatenate(runs)
-    pad = (-out.size) % 256
-    if pad:
-        out = np.concatenate([out, np.full(pad, -1)])
-    return out.astype(np.int32)
+    return np.concatenate(runs).astype(np.int32)
+
+
+def tiling(det_shape, tile=(16, 16)):
+    """A Tiling of the detector in ty x tx blocks (ty * tx in 64, 128, 256)."""
+    ty, tx = tile
+    if ty * tx not in TILE_SHAPES:
+        raise ValueError(f"tile {tile}: ty * tx must be 64, 128 or 256")
+    o = tile_order(det_shape, tile)
+    return None if o is None else Tiling(o, ty * tx, (int(ty), int(tx)))
 
 
 class NufftPlan:
@@ -195,6 +212,9 @@ class NufftPlan:
         q = q.to(self.device, torch.float64).contiguous()
         N_o = quats.shape[0]
         N_p = q.shape[1]
+        tile_points = 256
+        if isinstance(order, Tiling):
+            order, tile_points = order.order, order.points
         if order is not None:
             if isinstance(order, np.ndarray):
                 order = torch.as_tensor(order)
@@ -210,7 +230,8 @@ class NufftPlan:
             bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         with _span(timers, "nufft"):
             rc = self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, _ptr(order),
-                                           order.numel() if order is not None else 0, float(scale),
+                                           order.numel() if order is not None else 0, tile_points,
+                                           float(scale),
                                            _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
                                            _stream(stream))
         _lib.check(rc, "mtip_plan_slices")
@@ -262,7 +283,10 @@ def segment_scaling_ratio(img_sum, img_sumsq, offsets, sl_sum, sl_sumsq, n_pix, 
     """alpha per image: each segment (request) scaled by its own image std."""
     lib = _lib.load()
     dev = img_sum.device
-    off = torch.as_tensor(np.asarray(offsets, dtype=np.int64)).to(dev)
+    if torch.is_tensor(offsets) and offsets.device == dev and offsets.dtype == torch.int64:
+        off = offsets.contiguous()       # already resident (no copy, capturable)
+    else:
+        off = torch.as_tensor(np.asarray(offsets, dtype=np.int64)).to(dev)
     alpha = torch.empty(img_sum.numel(), dtype=torch.float64, device=dev)
     _lib.check(lib.mtip_segment_scaling_ratio(_ptr(img_sum), _ptr(img_sumsq), _ptr(off),
                                               off.numel() - 1, _ptr(sl_sum), _ptr(sl_sumsq),

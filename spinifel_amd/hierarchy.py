@@ -1,0 +1,293 @@
+"""Batching form: uneven image batches matched against a hierarchy of
+orientation grids, coarse to fine, entirely on the device.
+
+Levels are near-uniform quaternion grids of increasing size (1k, 10k, 100k
+by default).  Each node of level l + 1 takes as parents its ``n_parents``
+nearest nodes of level l (largest |q . q'|, found with the FP32 top-K
+kernel on zero-padded quaternions and their negatives); the children of a
+level-l node are the nodes that have it among their parents.  Nodes are
+stored in an order where siblings are adjacent and neighbouring parents are
+near in index (level 0 in rotation-vector Morton order, finer levels by
+parent), so the slices that concurrent tasks read lie close together.
+Children also include those of the node nearest each node's twin (the half
+turn about the beam axis after it): for a centrosymmetric intensity and a
+nearly flat detector cap the two give almost the same slice.
+
+A pass (``HierarchicalMatcher.run``):
+  * per batch (request) scaling by its own std ratio against level 0's
+    slices (``mtip_segment_scaling_ratio``), images centred by level 0's
+    mean slice and cast to bf16;
+  * dense top-T against level 0 (the MFMA top-K kernel), T = ``beam``;
+  * per finer level, (image, winner) entries grouped by winner
+    (``mtip_group_by_winner``), a persistent grouped top-K over the children
+    of each winner (``mtip_group_topk``) and a merge of each image's distinct
+    candidates (``mtip_merge_distinct``): top-T on middle levels, top-K on
+    the last;
+  * FP32 then FP64 rescoring of the last level's top-K (``mtip_rescore``).
+No step needs the host, so a pass over fixed batch sizes is captured once as
+a HIP graph (``capture``) and replayed (``replay``).
+
+The reference matches every image against one flat grid
+(spinifel/sequential/orientation_matching.py:189-256); this form is an
+addition beside it, and its result is checked against the flat search over
+the finest level.
+"""
+import numpy as np
+import torch
+
+from . import _lib, engine
+from .engine import _ptr, _stream
+from .quaternions import uniform_quaternions
+
+
+def _rotvec(q):
+    """Rotation vectors (axis * angle, |r| <= pi) of unit quaternions w >= 0."""
+    w = np.clip(q[:, 0], -1.0, 1.0)
+    v = q[:, 1:]
+    s = np.linalg.norm(v, axis=1)
+    ang = 2.0 * np.arctan2(s, w)
+    axis = v / np.maximum(s, 1e-300)[:, None]
+    return axis * ang[:, None]
+
+
+def morton_order(q, bits=10):
+    """Permutation sorting quaternions by the Morton code of their rotation vectors."""
+    r = _rotvec(np.asarray(q, np.float64))
+    u = np.clip(((r + np.pi) / (2.0 * np.pi) * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    code = np.zeros(len(u), dtype=np.int64)
+    for b in range(bits):
+        for d in range(3):
+            code |= ((u[:, d] >> b) & 1) << (3 * b + d)
+    return np.argsort(code, kind="stable")
+
+
+def children_csr(parents, n_coarse):
+    """CSR (offsets, child indices) of the coarse nodes from each fine node's parents.
+
+    parents: (N_fine, k) int array of coarse indices.  Children of a node are
+    in ascending fine index.
+    """
+    parents = np.asarray(parents, dtype=np.int64)
+    n_fine, k = parents.shape
+    p = parents.ravel()
+    c = np.repeat(np.arange(n_fine, dtype=np.int64), k)
+    order = np.lexsort((c, p))
+    counts = np.bincount(p, minlength=n_coarse)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return off, c[order].astype(np.int32)
+
+
+def nearest_parents(coarse, fine, k, device):
+    """Indices (N_fine, k) of each fine quaternion's k nearest coarse ones (GPU).
+
+    Nearest by |q_c . q_f| (q and -q are one rotation): the coarse set enters
+    with both signs, as rows of 64 FP32 columns (zero padded), so the FP32
+    top-K of ||s||^2 - 2 s . d = 1 - 2 q_c . q_f picks the largest dots.
+    """
+    dev = torch.device("cuda", device)
+    nc, nf = len(coarse), len(fine)
+    A = torch.zeros((2 * nc, 64), dtype=torch.float32, device=dev)
+    cq = torch.as_tensor(np.asarray(coarse, np.float32)).to(dev)
+    A[:nc, :4] = cq
+    A[nc:, :4] = -cq
+    B = torch.zeros((nf, 64), dtype=torch.float32, device=dev)
+    B[:, :4] = torch.as_tensor(np.asarray(fine, np.float32)).to(dev)
+    ones = torch.ones(2 * nc, dtype=torch.float64, device=dev)
+    S, D = engine.Operand(A, None, ones), engine.Operand(B)
+    idx, _ = engine.topk("fp32", S, ones, 2 * nc, D, nf, 64, engine._one(dev), k)
+    return (idx.cpu().numpy().astype(np.int64)) % nc
+
+
+def quat_mul(a, b):
+    """Hamilton products a * b of (n, 4) [w, x, y, z] arrays."""
+    w1, x1, y1, z1 = np.asarray(a, np.float64).T
+    w2, x2, y2, z2 = np.asarray(b, np.float64).T
+    return np.stack([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2],
+                    axis=1)
+
+
+def twin_quaternions(q, axis=(0.0, 0.0, 1.0)):
+    """q's twins: the half turn about the beam axis applied after q.
+
+    The model's intensity is centrosymmetric (a real autocorrelation) and a
+    detector near the beam samples an almost flat cap, so the slice at R
+    and at Rz(pi) R nearly coincide; refinement follows both branches."""
+    ax = np.asarray(axis, np.float64)
+    ax = ax / np.linalg.norm(ax)
+    h = np.tile(np.concatenate([[0.0], ax]), (len(q), 1))       # [cos(pi/2), sin(pi/2) axis]
+    t = quat_mul(h, q)
+    t[t[:, 0] < 0] *= -1.0
+    return t
+
+
+def union_csr(off, ch, partner):
+    """CSR whose row c is the union of rows c and partner[c] (ascending, distinct)."""
+    rows = [np.union1d(ch[off[c]:off[c + 1]], ch[off[p]:off[p + 1]]) for c, p in enumerate(partner)]
+    o = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    return o, (np.concatenate(rows) if rows else np.zeros(0)).astype(np.int32)
+
+
+class Hierarchy:
+    """Orientation grids of increasing size and their parent / child links.
+
+    twins: each coarse node's children also include those of the node
+    nearest its twin (``twin_quaternions`` about ``beam_axis``)."""
+
+    def __init__(self, sizes=(1000, 10000, 100000), n_parents=4, device=0, levels=None,
+                 twins=True, beam_axis=(0.0, 0.0, 1.0)):
+        if n_parents not in (1, 2, 4, 8):
+            raise ValueError("n_parents must be 1, 2, 4 or 8")
+        engine.require_gpu()
+        raw = [np.asarray(q, np.float64) for q in levels] if levels is not None else \
+            [uniform_quaternions(n) for n in sizes]
+        if len(raw) < 1:
+            raise ValueError("need at least one level")
+        self.device = torch.device("cuda", device)
+        self.n_parents = n_parents
+        q0 = raw[0][morton_order(raw[0])]
+        self.quats = [q0]
+        self.children = []          # per transition: (offsets, indices) numpy int32
+        for q in raw[1:]:
+            prev = self.quats[-1]
+            par = nearest_parents(prev, q, n_parents, device)
+            order = np.lexsort((morton_order(q).argsort(), par[:, 0]))
+            q, par = q[order], par[order]
+            self.quats.append(q)
+            off, ch = children_csr(par, len(prev))
+            if twins:
+                partner = nearest_parents(prev, twin_quaternions(prev, beam_axis), 1, device)[:, 0]
+                off, ch = union_csr(off, ch, partner)
+            self.children.append((off, ch))
+        self.sizes = [len(q) for q in self.quats]
+        dev = self.device
+        self.quats_dev = [torch.as_tensor(q).to(dev) for q in self.quats]
+        self.children_dev = [(torch.as_tensor(o).to(dev), torch.as_tensor(c).to(dev))
+                             for o, c in self.children]
+
+    @property
+    def n_levels(self):
+        return len(self.quats)
+
+
+class HierarchyModel:
+    """One model's slices at every level of a hierarchy, resident in HBM:
+    FP32 rows for rescoring and bf16 residuals (centred by level 0's mean
+    slice) for the top-K passes."""
+
+    def __init__(self, hierarchy, plan, q, scale, order=None, stream=None, timers=None):
+        self.h = hierarchy
+        self.levels = []
+        for hq in hierarchy.quats_dev:
+            sl = engine.SliceSet.compute(plan, hq, q, scale, stream=stream, timers=timers,
+                                         order=order)
+            self.levels.append(sl)
+        self.N_p = self.levels[0].N_p
+        self.center = self.levels[0].center
+        self.ops = [engine.residual(sl.f32, "bf16", sl.N_p, center=self.center, stream=stream)
+                    for sl in self.levels]
+        self.bad = sum(int(sl.bad.item()) for sl in self.levels)
+
+
+class HierarchicalMatcher:
+    """Refinement through a HierarchyModel's levels for a set of image batches.
+
+    beam: winners each image follows from one level to the next; K:
+    candidates of the last level rescored in FP32 / FP64."""
+
+    def __init__(self, model, K=8, beam=4):
+        if K not in (1, 2, 4, 8) or beam not in (1, 2, 4, 8):
+            raise ValueError("K and beam must be 1, 2, 4 or 8")
+        self.m = model
+        self.K = K
+        self.beam = beam
+        self.graph = None
+        self._static = None
+
+    def _pass(self, images, offsets, stream=None, timers=None, alpha=None):
+        lib = _lib.load()
+        m, h = self.m, self.m.h
+        dev = images.f32.device
+        N_i, N_p = images.N_i, images.N_p
+        L0 = m.levels[0]
+        st = _stream(stream)
+        with engine._span(timers, "scale"):
+            if alpha is None:
+                alpha = engine.segment_scaling_ratio(images.row_sum, images.row_sumsq, offsets,
+                                                     L0.row_sum, L0.row_sumsq, N_p, stream)
+            D = images.operand("bf16", alpha, m.center, stream)
+        last = h.n_levels - 1
+        T = self.beam
+        k0 = self.K if last == 0 else T
+        idx, val = engine.topk("bf16", m.ops[0], m.ops[0].sumsq, L0.N_o, D, N_i, N_p,
+                               engine._one(dev), k0, None, stream, timers)
+        levels = [idx]
+        ld = D.data.shape[1]
+        for l in range(1, h.n_levels):
+            n_groups = h.sizes[l - 1]
+            kl = self.K if l == last else T
+            nb = int(lib.mtip_group_workspace_size(N_i * T, n_groups))
+            ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+            with engine._span(timers, f"group{l}"):
+                _lib.check(lib.mtip_group_by_winner(_ptr(idx), idx.shape[1], T, N_i, n_groups,
+                                                    _ptr(ws), nb, st), "mtip_group_by_winner")
+            eidx = torch.full((N_i * T, kl), -1, dtype=torch.int32, device=dev)
+            evals = torch.full((N_i * T, kl), float("inf"), dtype=torch.float32, device=dev)
+            off, ch = h.children_dev[l - 1]
+            op = m.ops[l]
+            with engine._span(timers, f"level{l}"):
+                _lib.check(lib.mtip_group_topk(_ptr(ws), T, N_i, n_groups, _ptr(off), _ptr(ch),
+                                               _ptr(op.data), op.data.shape[1], _ptr(op.sumsq),
+                                               _ptr(D.data), ld, ld, kl, _ptr(eidx), _ptr(evals),
+                                               dev.index or 0, st), "mtip_group_topk")
+            idx = torch.empty((N_i, kl), dtype=torch.int32, device=dev)
+            val = torch.empty((N_i, kl), dtype=torch.float32, device=dev)
+            with engine._span(timers, f"merge{l}"):
+                _lib.check(lib.mtip_merge_distinct(_ptr(eidx), _ptr(evals), N_i, T * kl, kl,
+                                                   _ptr(idx), _ptr(val), st),
+                           "mtip_merge_distinct")
+            levels.append(idx)
+        r = engine.rescore(m.levels[last].f32, images.f32, N_p, alpha, idx, stream, timers)
+        return dict(index=r["best"], best_d64=r["best_d64"], topk_idx=idx, alpha=alpha,
+                    levels=levels)
+
+    def run(self, images, offsets, stream=None, timers=None, alpha=None):
+        """One eager pass.  images: engine.ImageSet of all batches concatenated;
+        offsets: batch boundaries (n_batches + 1); alpha: optional per-image
+        FP64 device scale replacing the per-batch std ratio.  Returns device
+        tensors: index (N_i,) int32 into the last level, best_d64, topk_idx,
+        alpha, levels (each level's top candidates)."""
+        return self._pass(images, offsets, stream, timers, alpha)
+
+    def capture(self, images, offsets, warmup=1):
+        """Capture one pass over these (resident) images as a HIP graph."""
+        offsets = torch.as_tensor(np.asarray(offsets, np.int64)).to(images.f32.device)
+        s = torch.cuda.Stream(device=images.f32.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                self._pass(images, offsets, s)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = self._pass(images, offsets)
+        self.graph, self._static = g, out
+        return out
+
+    def replay(self):
+        """Replay the captured pass; returns the captured output tensors."""
+        if self.graph is None:
+            raise RuntimeError("capture() first")
+        self.graph.replay()
+        return self._static
+
+    def orientations(self, index):
+        """Quaternions (N, 4) of last-level indices (host numpy)."""
+        ix = np.asarray(index.cpu().numpy() if torch.is_tensor(index) else index, np.int64)
+        return self.m.h.quats[-1][ix]
+
+
+def log_uniform_batches(n_batches, lo, hi, rng):
+    """Batch sizes floor(exp(U(ln lo, ln hi)))."""
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), int(n_batches)))).astype(np.int64)
