@@ -31,6 +31,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"hbm_gbs": 8000.0, "fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}   # GB/s, TFLOP/s
+_PREC_CODE = {"fp32": 0, "bf16": 1, "fp8": 2}
+
+
+def p_n_ref(W):
+    return int(W["p"]["n_ref"])
+
+
+def p_n_img(W):
+    return int(W["p"]["n_img"])
 
 
 def parse():
@@ -48,6 +57,15 @@ def parse():
     p.add_argument("--no-check", action="store_true", help="skip the accuracy checks")
     p.add_argument("--cpu-seconds", type=float, default=20.0,
                    help="target CPU time of the baseline sample")
+    p.add_argument("--no-tune", action="store_true", help="rule-based tiles, no autotuning")
+    p.add_argument("--no-batching", action="store_true", help="skip the batching form")
+    p.add_argument("--no-serving", action="store_true", help="skip the serving form")
+    p.add_argument("--batch-steps", type=int, default=5, help="timed graph replays per model")
+    p.add_argument("--serve-seconds", type=float, default=8.0,
+                   help="length of the Poisson arrival schedule")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                   help="only copy BYTES device-to-device 3 times (hardware-counter "
+                        "calibration of a plain copy of known size) and exit")
     return p.parse_args()
 
 
@@ -71,8 +89,13 @@ def build_workload(args, torch, engine, synth):
     plan.set_model(ac)
     q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
     scale = np.pi / E
+    # pixel tiling of the NUFFT: autotuned (or by rule)
+    if args.no_tune:
+        order, tune = engine.auto_tiling(plan, ppr.shape[1:]), None
+    else:
+        from spinifel_amd import autotune
+        order, tune = autotune.nufft_tiling(plan, q, np.pi / E, ppr.shape[1:])
     # images: GPU slices at the true orientations, clipped, scaled, Poisson (numpy, seeded)
-    order = torch.as_tensor(engine.tile_order(ppr.shape[1:])).to(dev)
     true_sl = plan.slices(torch.as_tensor(true).to(dev), q, scale, order=order)["s32"][:, :q.shape[1]]
     imgs = synth.poisson_images(true_sl.double().cpu().numpy(), p["mean_photons"],
                                 synth.rng_for(args.seed, synth.STREAM_POISSON))
@@ -80,7 +103,8 @@ def build_workload(args, torch, engine, synth):
     ac_dev = torch.as_tensor(ac).to(dev)
     quats_dev = torch.as_tensor(ref).to(dev)
     return dict(p=p, ac=ac, ac_dev=ac_dev, ppr=ppr, pdr=pdr, E=E, ref=ref, true=true, plan=plan,
-                q=q, scale=scale, quats=quats_dev, images=imgs, dev=dev, order=order)
+                q=q, scale=scale, quats=quats_dev, images=imgs, dev=dev, order=order,
+                nufft_tune=dict(tile=list(order.shape), ms=tune) if order is not None else None)
 
 
 def run_form(form, W, args, torch, engine, timers):
@@ -116,7 +140,29 @@ def run_form(form, W, args, torch, engine, timers):
         ms = timers.ms(name)
         kern[name] = float(np.mean(ms))
     out["kernel_ms"] = kern
+    # slices resident (serving / batching view): matching only
+    for _ in range(2):
+        engine.match_resident(sl, imgs, precision, K=K, rescore_fp64=resc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = engine.match_resident(sl, imgs, precision, K=K, rescore_fp64=resc)
+    torch.cuda.synchronize()
+    tr = (time.perf_counter() - t0) / args.steps
+    out["slices_resident"] = dict(images_per_s=N_i / tr, ms_per_match=1e3 * tr)
     return out, sl, r
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of each kernel from the committed counter summary
+    (profiles/traffic_<workload>.json: FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the counters' reading for a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch", {})
+    except (OSError, ValueError):
+        return {}
 
 
 def roofline_entries(W, kern_ms, forms, engine):
@@ -125,13 +171,14 @@ def roofline_entries(W, kern_ms, forms, engine):
     N_p = int(W["q"].shape[1])
     P = N_o * N_p
     npad = W["plan"].padded_edge
+    traffic = measured_traffic(W["workload"])
     ent = {}
     if "nufft" in kern_ms:
         # FP32 slice write + real fine grid read once + FP64 q + FP64 quats + FP64 row sums
         byts = 4 * P + 4 * npad ** 3 + 24 * N_p + 32 * N_o + 16 * N_o
         t = kern_ms["nufft"] * 1e-3
         ent["nufft"] = dict(bound="hbm", achieved=byts / t / 1e9, peak=PEAK["hbm_gbs"], unit="GB/s",
-                            frac=byts / t / 1e9 / PEAK["hbm_gbs"], traffic=None,
+                            frac=byts / t / 1e9 / PEAK["hbm_gbs"], traffic=traffic.get("nufft"),
                             algorithmic_bytes=byts, ms=kern_ms["nufft"],
                             taps_per_s=P * W["plan"].width ** 3 / t)
     for f in forms:
@@ -140,7 +187,8 @@ def roofline_entries(W, kern_ms, forms, engine):
             flops = 2.0 * N_o * N_i * N_p
             t = kern_ms[key] * 1e-3
             ent[key] = dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK[f], unit="TFLOP/s",
-                            frac=flops / t / 1e12 / PEAK[f], traffic=None, algorithmic_flops=flops,
+                            frac=flops / t / 1e12 / PEAK[f], traffic=traffic.get(key),
+                            algorithmic_flops=flops,
                             ms=kern_ms[key])
     return ent
 
@@ -234,6 +282,155 @@ def cpu_baseline(W, sl, args):
                 seconds_full_job=total)
 
 
+def gpu_images(plan, quats, q, scale, order, mean_photons, seed, torch):
+    """Poisson images (float32 device rows) at the given orientations: slices
+    clipped at 0, scaled to `mean_photons` per pixel, Poisson-sampled on the GPU."""
+    dev = plan.device
+    sl = plan.slices(torch.as_tensor(quats).to(dev), q, scale, order=order)["s32"][:, :q.shape[1]]
+    sl.clamp_(min=0.0)
+    sl.mul_(mean_photons / max(float(sl.mean()), 1e-30))
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    return torch.poisson(sl, generator=g)
+
+
+MODELS = {81: 800, 121: 2700}    # atoms per model size (same atom density)
+
+
+def run_batching(args, torch, engine, synth, log):
+    """64 batches, sizes log-uniform in [16, 4096], through a 1k / 10k / 100k
+    hierarchy, one captured HIP graph per model; agreement with the flat
+    search over the 100k level on an image subset."""
+    from spinifel_amd import autotune, hierarchy
+    from spinifel_amd.quaternions import random_quaternions
+    dev = torch.device("cuda", 0)
+    rng = synth.rng_for(args.seed, 7)
+    sizes = hierarchy.log_uniform_batches(64, 16, 4096, rng)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N_i = int(offs[-1])
+    ppr, pdr = synth.detector(4, 32, 128)
+    E = float(pdr.max())
+    q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
+    scale = np.pi / E
+    t0 = time.perf_counter()
+    h = hierarchy.Hierarchy(sizes=(1000, 10000, 100000), n_parents=4)
+    t_h = time.perf_counter() - t0
+    out = dict(n_batches=64, batch_sizes=sizes.tolist(), images=N_i, levels=h.sizes,
+               n_parents=4, beam=4, K=8, hierarchy_build_s=t_h, models={})
+    for M, atoms in MODELS.items():
+        log(f"batching: model M={M}")
+        ac = synth.make_model(M, atoms, args.seed)
+        plan = engine.NufftPlan(M, args.eps, 0)
+        plan.set_model(ac)
+        if args.no_tune:
+            tiling, tune = engine.auto_tiling(plan, ppr.shape[1:]), None
+        else:
+            tiling, tune = autotune.nufft_tiling(plan, q, scale, ppr.shape[1:])
+        t0 = time.perf_counter()
+        hm = hierarchy.HierarchyModel(h, plan, q, scale, order=tiling)
+        torch.cuda.synchronize()
+        t_sl = time.perf_counter() - t0
+        true = random_quaternions(N_i, synth.rng_for(args.seed + M, 8))
+        imgs = gpu_images(plan, true, q, scale, tiling, 1.0, args.seed + M, torch)
+        iset = engine.ImageSet(imgs)
+        del imgs
+        m = hierarchy.HierarchicalMatcher(hm, K=8, beam=4)
+        t = engine.Timers()
+        m.run(iset, offs, timers=t)            # eager pass: per-kernel times
+        torch.cuda.synchronize()
+        kern = {k: float(np.mean(t.ms(k))) for k in list(t.spans)}
+        m.capture(iset, offs)
+        for _ in range(2):
+            m.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            res = m.replay()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.batch_steps
+        # flat search over the finest level on a subset, same per-batch alpha
+        sub = np.sort(np.random.default_rng(11).choice(N_i, min(2048, N_i), replace=False))
+        subt = torch.as_tensor(sub).to(dev)
+        sset = engine.ImageSet(iset.f32[subt, :iset.N_p])
+        flat = engine.match_resident(hm.levels[-1], sset, "bf16", K=8,
+                                     alpha=res["alpha"][subt].contiguous())
+        got = res["index"][subt].cpu().numpy()
+        fl = flat["index"].cpu().numpy()
+        qh = h.quats[-1]
+        ang = lambda a, b: np.degrees(2 * np.arccos(np.clip(np.abs((a * b).sum(1)), 0, 1)))
+        out["models"][f"M{M}"] = dict(
+            images_per_s=N_i / dt, ms_per_pass=1e3 * dt, graph="hipGraph replay",
+            tiling=list(tiling.shape) if tiling is not None else None, nufft_tune_ms=tune,
+            slices_setup_s=t_sl, slices_resident=sum(h.sizes), kernel_ms_eager=kern,
+            agreement_vs_flat_100k=float((got == fl).mean()), flat_subset=int(len(sub)),
+            median_angle_to_true_deg=float(np.median(ang(qh[got], true[sub]))),
+            flat_median_angle_to_true_deg=float(np.median(ang(qh[fl], true[sub]))))
+        log("batching", f"M{M}", json.dumps({k: v for k, v in out["models"][f"M{M}"].items()
+                                             if k not in ("kernel_ms_eager", "nufft_tune_ms")}))
+        del hm, m, iset, sset, res, flat
+        torch.cuda.empty_cache()
+    return out
+
+
+def run_serving(args, torch, engine, synth, log):
+    """Four resident models (M = 81, 81, 121, 121 with 10k / 100k orientations),
+    Poisson bursts at 120/s of U[1, 512] images each, admitted by free HBM;
+    then the same bursts all at once (capacity)."""
+    from spinifel_amd import autotune, serving
+    from spinifel_amd.quaternions import random_quaternions
+    dev = torch.device("cuda", 0)
+    ppr, pdr = synth.detector(4, 32, 128)
+    E = float(pdr.max())
+    q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
+    scale = np.pi / E
+    srv = serving.Server(precision="bf16", K=8, n_slots=4, max_batch_images=4096)
+    pools = {}
+    spec = [("m81_10k", 81, 10000), ("m81_100k", 81, 100000), ("m121_10k", 121, 10000),
+            ("m121_100k", 121, 100000)]
+    t0 = time.perf_counter()
+    for k, (name, M, n_ref) in enumerate(spec):
+        ac = synth.make_model(M, MODELS[M], args.seed + k)
+        plan = engine.NufftPlan(M, args.eps, 0)
+        plan.set_model(ac)
+        tiling = engine.auto_tiling(plan, ppr.shape[1:]) if args.no_tune else \
+            autotune.nufft_tiling(plan, q, scale, ppr.shape[1:])[0]
+        ref = random_quaternions(n_ref, synth.rng_for(args.seed + k, 9))
+        srv.add_model(name, ac, ref, ppr, pdr, tiling=tiling)
+        true = random_quaternions(1024, synth.rng_for(args.seed + k, 10))
+        imgs = gpu_images(srv._plans[(M, srv.eps)], true, q, scale, tiling, 1.0, args.seed + k,
+                          torch)
+        pools[name] = imgs.cpu().numpy()
+        del imgs
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    rng = synth.rng_for(args.seed, 11)
+    names = [s[0] for s in spec]
+    sched = serving.poisson_bursts(120.0, args.serve_seconds, 512, names, rng)
+
+    # request images drawn up front (host), so submission costs no sampling
+    reqs = [pools[m][rng.integers(0, len(pools[m]), n)] for (_, m, n) in sched]
+    it = iter(range(len(reqs)))
+
+    def source(name, n):
+        return reqs[next(it)]
+
+    # warm-up: one request per model
+    for n in names:
+        srv.match_now(n, pools[n][:8])
+    st = serving.run_load(srv, sched, source)
+    srv.completed = []
+    # capacity: the same requests submitted at once
+    it = iter(range(len(reqs)))
+    sat = serving.run_load(srv, [(0.0, m, n) for (_, m, n) in sched], source)
+    free, total = serving.free_hbm(0)
+    out = dict(models=[dict(name=n, M=M, orientations=r) for n, M, r in spec],
+               setup_s=t_setup, arrival="Poisson 120 bursts/s, sizes U[1, 512]",
+               seconds=args.serve_seconds, poisson=st, saturated=sat,
+               hbm_free_gb=free / 1e9, hbm_total_gb=total / 1e9)
+    log("serving", json.dumps({"poisson": st, "saturated": sat}))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,11 +440,38 @@ def main():
     import torch
     from spinifel_amd import engine, synth
     engine.require_gpu()
+    if args.copy_probe:
+        n = int(args.copy_probe)
+        a = torch.empty(n // 4, dtype=torch.float32, device="cuda")
+        a.uniform_()
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n}), flush=True)
+        return
     forms = [f for f in args.forms.split(",") if f]
     log("building workload", args.workload)
     W = build_workload(args, torch, engine, synth)
+    W["workload"] = args.workload
     W["imgset"] = engine.ImageSet(W["images"].reshape(W["images"].shape[0], -1), W["dev"])
     torch.cuda.synchronize()
+    gemm_tune = {}
+    if not args.no_tune:
+        from spinifel_amd import autotune
+        log("autotuning GEMM tiles")
+        sl0 = engine.SliceSet.compute(W["plan"], W["quats"], W["q"], W["scale"], order=W["order"])
+        im = W["imgset"]
+        a0 = engine.scaling_ratio(im.row_sum, im.row_sumsq, sl0.row_sum, sl0.row_sumsq, im.N_p)
+        for f in forms:
+            if f == "fp32":
+                continue
+            S, D = sl0.operand(f, True), im.operand(f, a0, sl0.center)
+            rows, ms = autotune.gemm_tile(f, S, sl0.N_o, D, im.N_i, im.N_p, K=8)
+            gemm_tune[f] = dict(rows=rows, ms=ms)
+        del sl0, S, D
+        torch.cuda.empty_cache()
     res, last, timers = {}, {}, {}
     sl = None
     for f in forms:
@@ -278,6 +502,13 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(W, sl, args)
+    gemm_tile = {f: int(engine._lib.load().mtip_gemm_tile(_PREC_CODE[f], p_n_ref(W), p_n_img(W)))
+                 for f in forms}
+    del W["imgset"], sl, last
+    torch.cuda.empty_cache()
+    batching = None if args.no_batching else run_batching(args, torch, engine, synth, log)
+    torch.cuda.empty_cache()
+    serving_res = None if args.no_serving else run_serving(args, torch, engine, synth, log)
     p = W["p"]
     line = {
         "metric": "images/s oriented, 10k orients x 16k px: BF16+rescore vs FP32; p50/p99 latency",
@@ -306,6 +537,10 @@ def main():
         "nufft_rel_l2_vs_nudft": nufft_err,
         "accuracy_subset": acc.get("_subset") if acc else None,
         "cpu_baseline": cpu,
+        "tuning": {"nufft_tiling": W["nufft_tune"], "gemm_tile_rows": gemm_tile,
+                   "gemm_timings": gemm_tune},
+        "batching": batching,
+        "serving": serving_res,
     }
     print(json.dumps(line), flush=True)
 

@@ -136,6 +136,15 @@ def tiling(det_shape, tile=(16, 16)):
     return None if o is None else Tiling(o, ty * tx, (int(ty), int(tx)))
 
 
+def auto_tiling(plan, det_shape):
+    """Tiling by rule, without timing: 16 x 16 blocks while a block's fine-grid
+    box fits LDS (fine grids up to ~200 points per edge at 0.9 Nyquist), 8 x 8
+    blocks beyond.  The autotuner (spinifel_amd.autotune) picks by timing."""
+    if len(tuple(det_shape)) < 2:
+        return None
+    return tiling(det_shape, (16, 16) if plan.n_fine <= 200 else (8, 8))
+
+
 class NufftPlan:
     """Type-2 NUFFT plan of a cubic M^3 model (ES kernel, sigma = 2)."""
 
@@ -222,8 +231,15 @@ class NufftPlan:
         if ld is None:
             ld = padded(N_p)
         if out is None:
-            out = torch.zeros((N_o, ld), dtype=torch.float32, device=self.device)
-        bf = torch.zeros((N_o, ld), dtype=torch.int16, device=self.device) if want_bf16 else None
+            # the kernel writes every pixel column; only pitch padding needs zeros
+            out = torch.empty((N_o, ld), dtype=torch.float32, device=self.device)
+            if ld > N_p:
+                out[:, N_p:].zero_()
+        bf = None
+        if want_bf16:
+            bf = torch.empty((N_o, ld), dtype=torch.int16, device=self.device)
+            if ld > N_p:
+                bf[:, N_p:].zero_()
         rs = torch.empty(N_o, dtype=torch.float64, device=self.device)
         rq = torch.empty(N_o, dtype=torch.float64, device=self.device)
         if bad is None:

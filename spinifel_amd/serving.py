@@ -90,7 +90,9 @@ class Server:
 
     # ------------------------------------------------------------- models
     def add_model(self, name, ac, ref_quats, pixel_position_reciprocal, pixel_distance_reciprocal,
-                  oversampling=1):
+                  oversampling=1, tiling=None):
+        """Compute and keep the model's reference slices (tiling: an
+        engine.Tiling of the detector, default 16 x 16 blocks)."""
         ac = np.asarray(ac, np.float32)
         M = ac.shape[0]
         key = (M, self.eps)
@@ -100,9 +102,9 @@ class Server:
         plan.set_model(ac)
         ppr = np.asarray(pixel_position_reciprocal, np.float64)
         E = float(np.asarray(pixel_distance_reciprocal).max())
+        order = tiling if tiling is not None else engine.auto_tiling(plan, ppr.shape[1:])
         sl = engine.SliceSet.compute(plan, np.asarray(ref_quats, np.float64), ppr.reshape(3, -1),
-                                     np.pi / (E * oversampling),
-                                     order=engine.tile_order(ppr.shape[1:]))
+                                     np.pi / (E * oversampling), order=order)
         torch.cuda.synchronize(self.device)
         if int(sl.bad.item()) > 0:
             raise AssertionError("NU points outside [-3pi, 3pi)")

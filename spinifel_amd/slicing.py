@@ -32,7 +32,7 @@ def _slices_device(ac, ref_orientations, pixel_position_reciprocal, reciprocal_e
     if q.shape[1] != N_pixels:
         raise ValueError(f"N_pixels = {N_pixels} but the detector has {q.shape[1]} pixels")
     scale = np.pi / (float(reciprocal_extent) * float(oversampling))
-    r = plan.slices(quats, q, scale, ld=N_pixels, order=engine.tile_order(ppr.shape[1:]))
+    r = plan.slices(quats, q, scale, ld=N_pixels, order=engine.auto_tiling(plan, ppr.shape[1:]))
     if int(r["bad"].item()) > 0:
         raise AssertionError("NU points outside [-3pi, 3pi)")
     return r["s32"]

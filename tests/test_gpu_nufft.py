@@ -21,10 +21,10 @@ def torch_dev():
     return torch.device("cuda", 0)
 
 
-def _gpu_slices(ac, quats, ppr, E, eps=1e-6, os_=1, tiled=True):
+def _gpu_slices(ac, quats, ppr, E, eps=1e-6, os_=1, tile=(16, 16)):
     import torch
     from spinifel_amd import engine
-    order = engine.tile_order(ppr.shape[1:]) if tiled else None
+    order = engine.tiling(ppr.shape[1:], tile) if tile else None
     plan = engine.NufftPlan(ac.shape[0], eps, 0)
     plan.set_model(ac)
     q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1)))
@@ -34,16 +34,17 @@ def _gpu_slices(ac, quats, ppr, E, eps=1e-6, os_=1, tiled=True):
     return r["s32"][:, :n_p].double().cpu().numpy(), r
 
 
-@pytest.mark.parametrize("tiled", [True, False])
-def test_slices_m41_vs_nudft(torch_dev, tiled):
-    """8 x 8 pixel tiles (LDS-staged boxes) and index-order runs of 64 pixels
-    (boxes too big for LDS on this detector: direct-load fallback)."""
-    print(f"case: M=41 1x32x32, 64 orientations, tiled={tiled}", flush=True)
+@pytest.mark.parametrize("tile", [(16, 16), (8, 16), (8, 8), None])
+def test_slices_m41_vs_nudft(torch_dev, tile):
+    """16 x 16, 8 x 16 and 8 x 8 pixel tiles (LDS-staged boxes) and
+    index-order runs of 256 pixels (boxes too big for LDS on this detector:
+    direct-load fallback)."""
+    print(f"case: M=41 1x32x32, 64 orientations, tile={tile}", flush=True)
     ac = synth.make_model(41, 300, 0)
     ppr, pdr = synth.detector(1, 32, 32)
     E = pdr.max()
     quats = synth.random_quaternions(64, np.random.default_rng(5))
-    s, r = _gpu_slices(ac, quats, ppr, E, tiled=tiled)
+    s, r = _gpu_slices(ac, quats, ppr, E, tile=tile)
     H, K, L = normalized_points(quats, ppr, E)
     rng = np.random.default_rng(0)
     sub = rng.choice(H.size, 3000, replace=False)
@@ -72,6 +73,24 @@ def test_slices_m81_16k_px_vs_nudft(torch_dev):
     err = rel_l2(s.ravel()[sub], ref)
     print("rel_l2 vs NUDFT (M=81)", err)
     assert err <= TOL
+
+
+def test_slices_m121_16k_px_small_tiles(torch_dev):
+    """M = 121 (n = 250 fine grid): 16 x 16 boxes exceed LDS, 8 x 8 tiles stage."""
+    print("case: M=121 4x32x128, 4 orientations, 8x8 and 16x16 tiles", flush=True)
+    ac = synth.make_model(121, 1200, 1)
+    ppr, pdr = synth.detector(4, 32, 128)
+    E = pdr.max()
+    quats = synth.random_quaternions(4, np.random.default_rng(9))
+    s8, _ = _gpu_slices(ac, quats, ppr, E, tile=(8, 8))
+    s16, _ = _gpu_slices(ac, quats, ppr, E, tile=(16, 16))
+    H, K, L = normalized_points(quats, ppr, E)
+    sub = np.random.default_rng(2).choice(H.size, 800, replace=False)
+    ref = nudft_real(ac, H[sub], K[sub], L[sub])
+    assert rel_l2(s8.ravel()[sub], ref) <= TOL
+    assert rel_l2(s16.ravel()[sub], ref) <= TOL
+    # identical arithmetic per point whatever the tiling
+    np.testing.assert_array_equal(s8, s16)
 
 
 @pytest.mark.parametrize("M", [20, 33])
