@@ -116,13 +116,18 @@ int mtip_plan_execute(mtip_plan *plan, const double *x, const double *y, const d
  * of neighbouring pixels let the kernel stage one small box of the fine grid
  * per tile in LDS; smaller tiles keep the box of a large grid (M >= 121)
  * within LDS.  NULL visits pixels in index order, tile_points at a time.
+ * orient_order (nullable, device int32, a permutation of [0, N_o)): the
+ * order in which orientations are processed (e.g. a space-filling-curve
+ * order, so that concurrently resident workgroups share L2); rows of `out`
+ * stay indexed by orientation either way.
  * Replaces the batched loop of spinifel/sequential/orientation_matching.py:230-241
  * and gen_model_slices (spinifel/slicing.py:10-45).
  */
 int mtip_plan_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
                      int64_t N_p, const int32_t *pixel_order, int64_t n_order, int tile_points,
-                     double scale, float *out, int64_t ld_out, uint16_t *out_bf16,
-                     double *row_sum, double *row_sumsq, int *bad_count, void *stream);
+                     const int32_t *orient_order, double scale, float *out, int64_t ld_out,
+                     uint16_t *out_bf16, double *row_sum, double *row_sumsq, int *bad_count,
+                     void *stream);
 
 /*
  * Rotated, normalised q-points only: H/K/L (device FP64, N_o * N_p,
@@ -263,11 +268,15 @@ int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N
  * (lowest orientation index on ties).  cand: device int32 (N_i, K);
  * entries outside [0, N_o) are skipped (their d32 / d64 are NaN).  This is the distance of
  * spinifel/sequential/nearest_neighbor.py:134 (squared, before the sqrt).
+ * image_order (nullable, device int32, a permutation of [0, N_i)): the order
+ * in which images are processed (images with shared candidates next to each
+ * other reuse the candidates' slices from cache); outputs stay per image.
  */
 int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,
                  int64_t N_i, int64_t N_p, const double *alpha, int alpha_per_image,
                  const int32_t *cand, int K,
-                 int32_t *best_idx, double *best_d64, float *d32, double *d64, void *stream);
+                 int32_t *best_idx, double *best_d64, float *d32, double *d64,
+                 const int32_t *image_order, void *stream);
 
 /*
  * Full squared-distance matrix euDist[i * numRef + n] = ||data_i - ref_n||^2
@@ -334,6 +343,14 @@ int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, int64_t n_gro
                     const int32_t *child_off, const int32_t *child_idx, const uint16_t *slices,
                     int64_t ld_s, const double *slice_sumsq, const uint16_t *images, int64_t ld_d,
                     int64_t kd, int K, int32_t *top_idx, float *top_val, int device, void *stream);
+
+/*
+ * host: byte offset in the grouping workspace of the int32 entry permutation
+ * (N_e entries: each group's entries together, groups in order, entries
+ * with no valid winner last), valid after mtip_group_by_winner; usable as
+ * the image_order of mtip_rescore when n_win = 1.
+ */
+size_t mtip_group_perm_offset(int64_t N_e, int64_t n_groups);
 
 /* Number of entries the last mtip_group_by_winner left out (device int32 copy). */
 int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_i, int64_t n_groups,

@@ -39,8 +39,8 @@ _SIGS = {
                               _c.POINTER(_i32)]),
     "mtip_plan_set_model": (_i32, [_vp, _vp, _vp]),
     "mtip_plan_execute": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
-    "mtip_plan_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _f64, _vp, _i64, _vp,
-                                _vp, _vp, _vp, _vp]),
+    "mtip_plan_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp, _f64, _vp, _i64,
+                                _vp, _vp, _vp, _vp, _vp]),
     "mtip_rotate_positions": (_i32, [_vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "mtip_scaling_ratio": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
@@ -58,6 +58,7 @@ _SIGS = {
     "mtip_group_topk": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64,
                                _i32, _vp, _vp, _i32, _vp]),
     "mtip_group_bad_count": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp]),
+    "mtip_group_perm_offset": (_c.c_size_t, [_i64, _i64]),
     "mtip_merge_distinct": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp]),
     "mtip_set_gemm_tile": (_i32, [_i32, _i32]),
     "mtip_gemm_tile": (_i32, [_i32, _i64, _i64]),
@@ -65,7 +66,7 @@ _SIGS = {
                                   _vp, _i32, _vp, _c.c_size_t, _vp]),
     "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
     "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp,
-                            _vp, _vp, _vp, _vp]),
+                            _vp, _vp, _vp, _vp, _vp]),
     "mtip_euclidean_distance": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),
     "mtip_heap_sort": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp]),
 }

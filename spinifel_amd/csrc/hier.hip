@@ -5,20 +5,19 @@
 // finer nodes that have the winner among their nearest coarser nodes).
 // Images are grouped by winner on the device (count, scan, scatter, task
 // list), so the images of one group share one read of the children's slices;
-// a persistent kernel then walks the (group, 32-image chunk) tasks.  An
+// a persistent kernel then walks the (group, 64-entry chunk) tasks.  An
 // image may follow several winners (a beam of T): grouping then works on
 // entries (image, t), each entry gets its own top-K, and a merge keeps each
 // image's best distinct candidates.  No
 // step needs the host, so a whole multi-level pass can be captured in a
 // hipGraph and replayed.
 //
-// Grouped top-K block: 32 children (MFMA A rows) x 32 images (B rows) on a
-// v_mfma_f32_32x32x16_bf16; the four waves of a workgroup take interleaved
-// 64-pixel chunks and their partial products are summed in LDS.  Each
-// lane streams its own operand rows with 16-byte loads (rows are gathered,
-// so there is nothing to share through LDS).  Within each 64-element chunk
-// of pixels, lane half h feeds elements [32 h, 32 h + 32) in four MFMA
-// steps, for both operands alike, so every pixel meets itself.
+// Grouped top-K block: per wave 32 children (MFMA A rows) x 2 x 32 entries
+// (B rows) on v_mfma_f32_32x32x16_bf16.  Each lane streams its own operand
+// rows with 16-byte loads (rows are gathered, so there is nothing to share
+// through LDS).  Within each 64-element chunk of pixels, lane half h feeds
+// elements [32 h, 32 h + 32) in four MFMA steps, for both operands alike,
+// so every pixel meets itself.
 #include "mtip_internal.h"
 
 namespace {
@@ -26,8 +25,7 @@ namespace {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kTaskImages = 32;     // images per task (one MFMA block of columns)
-constexpr int kChunk = 32;          // children per MFMA block of rows
+constexpr int kTaskImages = 64;     // entries per task (two MFMA blocks of columns)
 constexpr uint64_t KEY_NONE = ~0ull;
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
@@ -88,12 +86,14 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ count
     }
     if (tid == 1023) {
         img_off[n_groups] = s_a[1023];
+        cursor[n_groups] = s_a[1023];     // entries left out go after the grouped ones
         n_tasks[0] = s_b[1023];
     }
 }
 
 // perm[pos] = entry, the entries of a group contiguous (order within a group
-// is arbitrary; every per-entry result is independent of it).
+// is arbitrary; every per-entry result is independent of it), entries with
+// no valid winner at the end: perm is a permutation of all entries.
 __global__ void k_scatter(const int32_t *__restrict__ win, int64_t win_stride, int64_t n_win,
                           int64_t N_e, int64_t n_groups, int32_t *__restrict__ cursor,
                           int32_t *__restrict__ perm)
@@ -101,7 +101,8 @@ __global__ void k_scatter(const int32_t *__restrict__ win, int64_t win_stride, i
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N_e) return;
     const int32_t g = win[(e / n_win) * win_stride + e % n_win];
-    if (g >= 0 && g < n_groups) perm[atomicAdd(&cursor[g], 1)] = (int32_t)e;
+    const int64_t slot = (g >= 0 && g < n_groups) ? g : n_groups;
+    perm[atomicAdd(&cursor[slot], 1)] = (int32_t)e;
 }
 
 // tasks[task_off[g] + t] = (g, first image position, end position)
@@ -128,10 +129,15 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 }
 
 // Per task: the K children of the group with the smallest
-//   v = ||s_c||^2 - 2 s_c . d_i
-// for each of up to 32 images (the centred, scaled operands of the dense
+//   v = ||s_c||^2 - 2 s_c . d_e
+// for each of up to 64 entries (the centred, scaled operands of the dense
 // path, so v ranks like the squared distance).  kd: pixels summed, a
 // multiple of 64 within both row pitches; columns past N_p are zero.
+//
+// Wave w takes children c0 + 32 w .. c0 + 32 w + 31 of each pass of 128
+// and the full pixel range, for two 32-entry column blocks, so a child's
+// row is read once per task; the waves' running top-K lists are merged in
+// LDS at the end.
 template <int K>
 __global__ __launch_bounds__(256) void k_group_topk(
     const int4 *__restrict__ tasks, const int32_t *__restrict__ n_tasks,
@@ -140,95 +146,116 @@ __global__ __launch_bounds__(256) void k_group_topk(
     const double *__restrict__ s_sumsq, const uint16_t *__restrict__ D, int64_t ldd, int64_t kd,
     int n_win, int32_t *__restrict__ out_idx, float *__restrict__ out_val)
 {
-    __shared__ float s_part[3][64][16];
+    __shared__ uint64_t s_top[4][kTaskImages][K];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
-    // wave w takes the 64-element chunks w, w + 4, w + 8, ... of [0, kd)
-    const int64_t k0 = 64 * wave;
     const int nt = n_tasks[0];
 
     for (int task = blockIdx.x; task < nt; task += gridDim.x) {
         const int4 tk = tasks[task];
         const int g = tk.x, p0 = tk.y, p1 = tk.z;
         const int c_begin = child_off[g], c_end = child_off[g + 1];
-        // this lane's image row (B operand column l32): entry e is image e / n_win
-        const uint16_t *drow =
-            (p0 + l32 < p1) ? D + (int64_t)(perm[p0 + l32] / n_win) * ldd : nullptr;
-        uint64_t top[K];
+        // this lane's entry rows (B operand column l32 of blocks 0 and 1)
+        const uint16_t *drow[2];
 #pragma unroll
-        for (int r = 0; r < K; ++r) top[r] = KEY_NONE;
+        for (int j = 0; j < 2; ++j) {
+            const int pe = p0 + 32 * j + l32;
+            drow[j] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
+        }
+        uint64_t top[2][K];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < K; ++r) top[j][r] = KEY_NONE;
 
-        for (int c0 = c_begin; c0 < c_end; c0 += kChunk) {
+        for (int c0 = c_begin + 32 * wave; c0 < c_end; c0 += 128) {
             const int crow = c0 + l32;
             const int child = crow < c_end ? child_idx[crow] : -1;
             const uint16_t *srow = child >= 0 ? S + (int64_t)child * lds : nullptr;
-            floatx16 acc;
+            floatx16 acc[2];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-            // 64-element chunks: loads of the next chunk in flight during this one's MFMAs
-            uint4 a[4], b[4];
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
+            // 64-element chunks, the next chunk's loads in flight during this one's MFMAs
+            uint4 a[4], b0[4], b1[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                a[s] = k0 < kd ? ld16(srow, k0 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
-                b[s] = k0 < kd ? ld16(drow, k0 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                const int64_t e = half * 32 + 8 * s;
+                a[s] = ld16(srow, e);
+                b0[s] = ld16(drow[0], e);
+                b1[s] = ld16(drow[1], e);
             }
-            for (int64_t k = k0; k < kd; k += 256) {
-                uint4 an[4], bn[4];
-                const bool more = k + 256 < kd;
+            for (int64_t k = 0; k < kd; k += 64) {
+                uint4 an[4], bn0[4], bn1[4];
+                const bool more = k + 64 < kd;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    an[s] = more ? ld16(srow, k + 256 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
-                    bn[s] = more ? ld16(drow, k + 256 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                    const int64_t e = k + 64 + half * 32 + 8 * s;
+                    an[s] = more ? ld16(srow, e) : make_uint4(0, 0, 0, 0);
+                    bn0[s] = more ? ld16(drow[0], e) : make_uint4(0, 0, 0, 0);
+                    bn1[s] = more ? ld16(drow[1], e) : make_uint4(0, 0, 0, 0);
                 }
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b[s]),
-                                                                  acc, 0, 0, 0);
+                for (int s = 0; s < 4; ++s) {
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b0[s]),
+                                                                     acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b1[s]),
+                                                                     acc[1], 0, 0, 0);
+                }
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     a[s] = an[s];
-                    b[s] = bn[s];
+                    b0[s] = bn0[s];
+                    b1[s] = bn1[s];
                 }
             }
-            // sum the four waves' partial products in wave 0
-            if (wave > 0)
+            // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
 #pragma unroll
-                for (int v = 0; v < 16; ++v) s_part[wave - 1][lane][v] = acc[v];
-            __syncthreads();
-            if (wave == 0) {
+            for (int v = 0; v < 16; ++v) {
+                const int r = c0 + 8 * (v >> 2) + 4 * half + (v & 3);
+                if (r < c_end) {
+                    const int cid = child_idx[r];
+                    const float ss = (float)s_sumsq[cid];
 #pragma unroll
-                for (int w = 0; w < 3; ++w)
-#pragma unroll
-                    for (int v = 0; v < 16; ++v) acc[v] += s_part[w][lane][v];
-                // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int r = c0 + 8 * (v >> 2) + 4 * half + (v & 3);
-                    if (r < c_end) {
-                        const int cid = child_idx[r];
-                        const float val = fmaf(-2.0f, acc[v], (float)s_sumsq[cid]);
-                        mtip::topk_insert<K>(top, mtip::make_key(val, (uint32_t)cid));
-                    }
+                    for (int j = 0; j < 2; ++j)
+                        mtip::topk_insert<K>(top[j], mtip::make_key(fmaf(-2.0f, acc[j][v], ss),
+                                                                    (uint32_t)cid));
                 }
             }
-            __syncthreads();
         }
-        if (wave == 0) {
+        // merge the lane halves, then the four waves through LDS
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
             uint64_t other[K];
 #pragma unroll
-            for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[r], 32);
-            mtip::topk_merge<K>(top, other);
-            if (half == 0 && p0 + l32 < p1) {
-                const int64_t img = perm[p0 + l32];     // the entry (its own output row)
+            for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[j][r], 32);
+            mtip::topk_merge<K>(top[j], other);
+            if (half == 0)
 #pragma unroll
-                for (int r = 0; r < K; ++r) {
-                    const bool ok = top[r] != KEY_NONE;
-                    out_idx[img * K + r] = ok ? (int32_t)(uint32_t)top[r] : -1;
-                    out_val[img * K + r] =
-                        ok ? mtip::ordered_to_float((uint32_t)(top[r] >> 32)) : __builtin_inff();
-                }
+                for (int r = 0; r < K; ++r) s_top[wave][32 * j + l32][r] = top[j][r];
+        }
+        __syncthreads();
+        if (tid < kTaskImages && p0 + tid < p1) {
+            uint64_t t[K], o[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r) t[r] = s_top[0][tid][r];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+#pragma unroll
+                for (int r = 0; r < K; ++r) o[r] = s_top[w][tid][r];
+                mtip::topk_merge<K>(t, o);
+            }
+            const int64_t ent = perm[p0 + tid];     // the entry (its own output row)
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const bool ok = t[r] != KEY_NONE;
+                out_idx[ent * K + r] = ok ? (int32_t)(uint32_t)t[r] : -1;
+                out_val[ent * K + r] =
+                    ok ? mtip::ordered_to_float((uint32_t)(t[r] >> 32)) : __builtin_inff();
             }
         }
+        __syncthreads();
     }
 }
 
@@ -306,6 +333,12 @@ GroupWs carve(void *ws, int64_t N_i, int64_t n_groups)
     return w;
 }
 }  // namespace
+
+extern "C" size_t mtip_group_perm_offset(int64_t N_e, int64_t n_groups)
+{
+    (void)N_e;
+    return (size_t)(4 * (n_groups + 1) + 2) * sizeof(int32_t);
+}
 
 extern "C" int mtip_group_by_winner(const int32_t *winners, int64_t win_stride, int n_win,
                                     int64_t N_i, int64_t n_groups, void *workspace,

@@ -553,11 +553,14 @@ __global__ __launch_bounds__(256) void k_rescore(const float *__restrict__ S, in
                                                  const int32_t *__restrict__ cand,
                                                  int32_t *__restrict__ best_idx,
                                                  double *__restrict__ best_d64,
-                                                 float *__restrict__ d32, double *__restrict__ d64)
+                                                 float *__restrict__ d32, double *__restrict__ d64,
+                                                 const int32_t *__restrict__ order)
 {
     __shared__ double s64[4][K];
     __shared__ float s32[4][K];
-    const int64_t i = blockIdx.x;
+    // images in the caller's order (e.g. grouped by best candidate, so that
+    // neighbouring workgroups read the same candidate slices)
+    const int64_t i = order ? (int64_t)order[blockIdx.x] : (int64_t)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double al = alpha_per_image ? alpha[i] : alpha[0];
     const float alf = (float)al;
@@ -1033,7 +1036,7 @@ extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, cons
                             int64_t N_i, int64_t N_p, const double *alpha, int alpha_per_image,
                             const int32_t *cand,
                             int K, int32_t *best_idx, double *best_d64, float *d32, double *d64,
-                            void *stream)
+                            const int32_t *image_order, void *stream)
 {
     MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_rescore: K must be 1, 2, 4 or 8");
     MTIP_REQUIRE(N_i >= 0 && N_p >= 0, MTIP_ERR_ARG, "mtip_rescore: negative size");
@@ -1044,10 +1047,10 @@ extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, cons
     hipStream_t st = (hipStream_t)stream;
     dim3 g((unsigned)N_i);
     switch (K) {
-    case 1: hipLaunchKernelGGL(k_rescore<1>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
-    case 2: hipLaunchKernelGGL(k_rescore<2>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
-    case 4: hipLaunchKernelGGL(k_rescore<4>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
-    default: hipLaunchKernelGGL(k_rescore<8>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64); break;
+    case 1: hipLaunchKernelGGL(k_rescore<1>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64, image_order); break;
+    case 2: hipLaunchKernelGGL(k_rescore<2>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64, image_order); break;
+    case 4: hipLaunchKernelGGL(k_rescore<4>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64, image_order); break;
+    default: hipLaunchKernelGGL(k_rescore<8>, g, dim3(256), 0, st, slices, ld_s, N_o, images, ld_d, N_p, alpha, alpha_per_image, cand, best_idx, best_d64, d32, d64, image_order); break;
     }
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;

@@ -607,7 +607,9 @@ template <int W, int TP>
 __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double *__restrict__ quats,
                                                     const double *__restrict__ q, int64_t N_p,
                                                     const int32_t *__restrict__ order,
-                                                    int64_t n_order, double gscale,
+                                                    int64_t n_order,
+                                                    const int32_t *__restrict__ orient_order,
+                                                    double gscale,
                                                     float *__restrict__ out, int64_t ld_out,
                                                     uint16_t *__restrict__ out_bf16,
                                                     double *__restrict__ row_sum,
@@ -628,7 +630,12 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane / GP, c = lane % GP;
-    const int64_t o = blockIdx.x;
+    // Workgroups go round-robin over the 8 XCDs; give each XCD a contiguous
+    // run of the (caller-ordered) orientations so that the ones resident on
+    // an XCD at the same time are neighbours and share its L2.
+    const int64_t nb = gridDim.x, n8 = nb / 8, b = blockIdx.x;
+    const int64_t slot = b < 8 * n8 ? (b % 8) * n8 + b / 8 : b;
+    const int64_t o = orient_order ? (int64_t)orient_order[slot] : slot;
     // R^-1 of this orientation, kept in LDS: as registers its nine doubles
     // would be live across the whole kernel
     __shared__ double s_ri[9];
@@ -687,12 +694,6 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
             double X2 = (s_ri[6] * x + s_ri[7] * y + s_ri[8] * z) * gscale;
             ok = point_corner<W>(gv, X0, X1, X2, ijk, t);
         }
-        if (has_pt) {
-            s_pix[tid] = pix;
-            s_t[0][tid] = t[0];
-            s_t[1][tid] = t[1];
-            s_t[2][tid] = t[2];
-        }
         // ---- box of the tile's windows (block min / max)
         {
             int v[6] = {ok ? ijk[0] : 0x7fffffff, ok ? ijk[1] : 0x7fffffff, ok ? ijk[2] : 0x7fffffff,
@@ -717,16 +718,23 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
         const int bz = any ? hi[2] - lz + 2 * GP : 0;
         const int S = pair_stride(bz);
         const bool staged = any && S > 0 && bx * by * S <= kBoxCap;
-        {
-            int lb = -1;
-            if (ok) {
-                const int zb = ijk[2] & ~1;
-                lb = staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * S + (zb - lz)
-                            : (ijk[0] * np + ijk[1]) * np + zb;
-                lb |= ijk[2] & 1;
-            }
-            if (has_pt) s_lb[tid] = lb;
+        // this point's pair base (box-local when staged, padded grid otherwise)
+        int lb = -1;
+        if (ok) {
+            const int zb = ijk[2] & ~1;
+            lb = staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * S + (zb - lz)
+                        : (ijk[0] * np + ijk[1]) * np + zb;
+            lb |= ijk[2] & 1;
         }
+
+        if (has_pt) {
+            s_lb[tid] = lb;
+            s_pix[tid] = pix;
+            s_t[0][tid] = t[0];
+            s_t[1][tid] = t[1];
+            s_t[2][tid] = t[2];
+        }
+        // ---- the box
         if (staged) {
             // the box as nrows * q4 chunks of 16 bytes, chunk k = row k / q4,
             // quarter k % q4: every thread's loads go in flight at once (at most
@@ -854,13 +862,13 @@ __global__ __launch_bounds__(256) void k_positions(const double *__restrict__ qu
 
 template <int W>
 void launch_slices(const GridView &gv, const double *quats, int64_t N_o, const double *q,
-                   int64_t N_p, const int32_t *order, int64_t n_order, int tile, double gscale,
-                   float *out, int64_t ld, uint16_t *obf, double *rs, double *rq, int *bad,
-                   hipStream_t st)
+                   int64_t N_p, const int32_t *order, int64_t n_order, int tile,
+                   const int32_t *orient_order, double gscale, float *out, int64_t ld,
+                   uint16_t *obf, double *rs, double *rq, int *bad, hipStream_t st)
 {
 #define MTIP_SLICES_TP(T)                                                                     \
     hipLaunchKernelGGL((k_slices_lds<W, T>), dim3((unsigned)N_o), dim3(256), 0, st, gv, quats, q, \
-                       N_p, order, n_order, gscale, out, ld, obf, rs, rq, bad)
+                       N_p, order, n_order, orient_order, gscale, out, ld, obf, rs, rq, bad)
     if (tile == 64) MTIP_SLICES_TP(64);
     else if (tile == 128) MTIP_SLICES_TP(128);
     else MTIP_SLICES_TP(256);
@@ -1062,7 +1070,8 @@ extern "C" int mtip_plan_execute(mtip_plan *p, const double *x, const double *y,
 
 extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, const double *q,
                                 int64_t N_p, const int32_t *pixel_order, int64_t n_order,
-                                int tile_points, double scale, float *out, int64_t ld_out,
+                                int tile_points, const int32_t *orient_order, double scale,
+                                float *out, int64_t ld_out,
                                 uint16_t *out_bf16, double *row_sum, double *row_sumsq,
                                 int *bad_count, void *stream)
 {
@@ -1085,8 +1094,8 @@ extern "C" int mtip_plan_slices(mtip_plan *p, const double *quats, int64_t N_o, 
     // grid units per q unit: (pi / (E os)) * n / (2 pi)
     double gscale = scale * (double)p->n / (2.0 * M_PI);
     MTIP_W_DISPATCH(p->w, launch_slices, gv, quats, N_o, q, N_p, pixel_order, n_order,
-                    tile_points, gscale, out, ld_out, out_bf16, row_sum, row_sumsq, bad_count,
-                    (hipStream_t)stream);
+                    tile_points, orient_order, gscale, out, ld_out, out_bf16, row_sum, row_sumsq,
+                    bad_count, (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }

@@ -136,6 +136,25 @@ def tiling(det_shape, tile=(16, 16)):
     return None if o is None else Tiling(o, ty * tx, (int(ty), int(tx)))
 
 
+_ORIENT_CACHE = {}
+
+
+def _orient_order(quats):
+    """Morton order of the orientations of a device quaternion buffer, cached
+    by buffer address and size (a stale order after the buffer is rewritten
+    changes only the speed, not the slices)."""
+    key = (quats.data_ptr(), quats.shape[0], str(quats.device))
+    hit = _ORIENT_CACHE.get(key)
+    if hit is None:
+        from .quaternions import morton_order
+        perm = morton_order(quats.cpu().numpy())
+        hit = torch.as_tensor(perm.astype(np.int32)).to(quats.device)
+        if len(_ORIENT_CACHE) > 64:
+            _ORIENT_CACHE.clear()
+        _ORIENT_CACHE[key] = hit
+    return hit
+
+
 def auto_tiling(plan, det_shape):
     """Tiling by rule, without timing: 16 x 16 blocks while a block's fine-grid
     box fits LDS (fine grids up to ~200 points per edge at 0.9 Nyquist), 8 x 8
@@ -209,13 +228,16 @@ class NufftPlan:
         return out
 
     def slices(self, quats, q, scale, out=None, ld=None, want_bf16=False, bad=None, stream=None,
-               timers=None, order=None):
+               timers=None, order=None, orient_order="auto"):
         """Slices at N_o orientations for pixel q-vectors q (3, N_p).
 
         quats: (N_o, 4) float64 device tensor; q: (3, N_p) float64 device tensor.
         Returns dict(s32 (N_o, ld) float32, bf16 (N_o, ld) int16 bits or None,
         row_sum, row_sumsq (N_o,) float64, bad (1,) int32).  Pixel columns
-        beyond N_p are zero.
+        beyond N_p are zero.  order: a Tiling (or pixel order in runs of 256);
+        orient_order: processing order of the orientations ('auto': Morton
+        order of their rotation vectors, cached per quaternion buffer; None:
+        index order; or a device int32 permutation).
         """
         quats = quats.to(self.device, torch.float64).contiguous()
         q = q.to(self.device, torch.float64).contiguous()
@@ -244,10 +266,12 @@ class NufftPlan:
         rq = torch.empty(N_o, dtype=torch.float64, device=self.device)
         if bad is None:
             bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        if isinstance(orient_order, str):
+            orient_order = _orient_order(quats) if N_o >= 64 else None
         with _span(timers, "nufft"):
             rc = self.lib.mtip_plan_slices(self._h, _ptr(quats), N_o, _ptr(q), N_p, _ptr(order),
                                            order.numel() if order is not None else 0, tile_points,
-                                           float(scale),
+                                           _ptr(orient_order), float(scale),
                                            _ptr(out), ld, _ptr(bf), _ptr(rs), _ptr(rq), _ptr(bad),
                                            _stream(stream))
         _lib.check(rc, "mtip_plan_slices")
@@ -414,8 +438,9 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
     return idx, val
 
 
-def rescore(S32, D32, N_p, alpha, cand, stream=None, timers=None):
-    """FP32 and FP64 distances of the candidates; FP64 argmin per image."""
+def rescore(S32, D32, N_p, alpha, cand, stream=None, timers=None, image_order=None):
+    """FP32 and FP64 distances of the candidates; FP64 argmin per image.
+    image_order: optional device int32 processing order of the images."""
     lib = _lib.load()
     N_i, K = cand.shape
     dev = cand.device
@@ -427,7 +452,7 @@ def rescore(S32, D32, N_p, alpha, cand, stream=None, timers=None):
         rc = lib.mtip_rescore(_ptr(S32), S32.shape[1], S32.shape[0], _ptr(D32), D32.shape[1], N_i,
                               N_p, _ptr(alpha), int(alpha.numel() > 1), _ptr(cand.contiguous()), K,
                               _ptr(best), _ptr(bd),
-                              _ptr(d32), _ptr(d64), _stream(stream))
+                              _ptr(d32), _ptr(d64), _ptr(image_order), _stream(stream))
     _lib.check(rc, "mtip_rescore")
     return dict(best=best, best_d64=bd, d32=d32, d64=d64)
 

@@ -37,28 +37,7 @@ import torch
 
 from . import _lib, engine
 from .engine import _ptr, _stream
-from .quaternions import uniform_quaternions
-
-
-def _rotvec(q):
-    """Rotation vectors (axis * angle, |r| <= pi) of unit quaternions w >= 0."""
-    w = np.clip(q[:, 0], -1.0, 1.0)
-    v = q[:, 1:]
-    s = np.linalg.norm(v, axis=1)
-    ang = 2.0 * np.arctan2(s, w)
-    axis = v / np.maximum(s, 1e-300)[:, None]
-    return axis * ang[:, None]
-
-
-def morton_order(q, bits=10):
-    """Permutation sorting quaternions by the Morton code of their rotation vectors."""
-    r = _rotvec(np.asarray(q, np.float64))
-    u = np.clip(((r + np.pi) / (2.0 * np.pi) * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
-    code = np.zeros(len(u), dtype=np.int64)
-    for b in range(bits):
-        for d in range(3):
-            code |= ((u[:, d] >> b) & 1) << (3 * b + d)
-    return np.argsort(code, kind="stable")
+from .quaternions import _rotvec, morton_order, uniform_quaternions  # noqa: F401
 
 
 def children_csr(parents, n_coarse):
@@ -247,7 +226,18 @@ class HierarchicalMatcher:
                                                    _ptr(idx), _ptr(val), st),
                            "mtip_merge_distinct")
             levels.append(idx)
-        r = engine.rescore(m.levels[last].f32, images.f32, N_p, alpha, idx, stream, timers)
+        # rescore with the images grouped by their best candidate, so that
+        # neighbouring workgroups read the same candidate slices
+        n_last = h.sizes[last]
+        nb = int(lib.mtip_group_workspace_size(N_i, n_last))
+        ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        with engine._span(timers, "order"):
+            _lib.check(lib.mtip_group_by_winner(_ptr(idx), idx.shape[1], 1, N_i, n_last, _ptr(ws),
+                                                nb, st), "mtip_group_by_winner")
+        po = int(lib.mtip_group_perm_offset(N_i, n_last))
+        order = ws[po:po + 4 * N_i].view(torch.int32)
+        r = engine.rescore(m.levels[last].f32, images.f32, N_p, alpha, idx, stream, timers,
+                           image_order=order)
         return dict(index=r["best"], best_d64=r["best_d64"], topk_idx=idx, alpha=alpha,
                     levels=levels)
 

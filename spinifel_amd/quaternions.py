@@ -37,3 +37,24 @@ def random_quaternions(n, rng):
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     q[q[:, 0] < 0] *= -1.0
     return q
+
+
+def _rotvec(q):
+    """Rotation vectors (axis * angle, |r| <= pi) of unit quaternions w >= 0."""
+    w = np.clip(q[:, 0], -1.0, 1.0)
+    v = q[:, 1:]
+    s = np.linalg.norm(v, axis=1)
+    ang = 2.0 * np.arctan2(s, w)
+    axis = v / np.maximum(s, 1e-300)[:, None]
+    return axis * ang[:, None]
+
+
+def morton_order(q, bits=10):
+    """Permutation sorting quaternions by the Morton code of their rotation vectors."""
+    r = _rotvec(np.asarray(q, np.float64))
+    u = np.clip(((r + np.pi) / (2.0 * np.pi) * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    code = np.zeros(len(u), dtype=np.int64)
+    for b in range(bits):
+        for d in range(3):
+            code |= ((u[:, d] >> b) & 1) << (3 * b + d)
+    return np.argsort(code, kind="stable")

@@ -54,6 +54,13 @@ def test_group_topk_equals_brute_force():
     _lib.check(lib.mtip_group_bad_count(_ptr(ws), 1, N_i, n_c, _ptr(bad), engine._stream()), "bad")
     torch.cuda.synchronize()
     assert int(bad.item()) == 5
+    # the entry permutation: every image once, grouped by winner, left-out ones last
+    po = int(lib.mtip_group_perm_offset(N_i, n_c))
+    perm = ws[po:po + 4 * N_i].view(torch.int32).cpu().numpy()
+    assert sorted(perm.tolist()) == list(range(N_i))
+    g_of = win[perm[:N_i - 5]]
+    assert np.all(np.diff(g_of) >= 0)
+    assert set(perm[N_i - 5:].tolist()) == set(range(5))
     idx, val = idx.cpu().numpy(), val.cpu().numpy()
     Sb, Db = _bf16_round(S32), _bf16_round(D32)
     ss = So.sumsq.cpu().numpy()
