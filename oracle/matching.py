@@ -147,6 +147,40 @@ def slicing_and_match(ac, slices_, pixel_position_reciprocal, pixel_distance_rec
     return ref_orientations[index]
 
 
+def snm_slicing_and_match(ac, slices_, pixel_position_reciprocal, pixel_distance_reciprocal,
+                          ref_orientations, N_batch_size, oversampling=1, eps=1e-12,
+                          return_details=False):
+    """Restatement of the class path SNM.slicing_and_match
+    (sequential/orientation_matching.py:122-186, with NUFFT.forward of
+    extern/nufft_ext.py:248-284): per batch of N_batch_size orientations the
+    model slices are scaled by ``slices_.std() / batch.std()`` (:165), then
+    squared distances ``||x||^2 + ||y||^2 - 2 x.y`` (:99-108) and the argmin
+    over all orientations (:179)."""
+    slices_ = np.asarray(slices_)
+    N_slices = slices_.shape[0]
+    if not N_slices:
+        return np.zeros((0, 4))
+    ref_orientations = np.asarray(ref_orientations)
+    N_o = ref_orientations.shape[0]
+    assert N_o % N_batch_size == 0, "N_orientations must be divisible by N_batch_size"
+    data = slices_.reshape(N_slices, -1).astype(np.float64)
+    std = data.std()
+    E = np.asarray(pixel_distance_reciprocal).max()
+    ms = model_slices(ac, ref_orientations, pixel_position_reciprocal, E, oversampling,
+                      N_batch_size, eps)
+    mult = np.empty(N_o)
+    for b in range(N_o // N_batch_size):
+        st, en = b * N_batch_size, (b + 1) * N_batch_size
+        mult[st:en] = std / ms[st:en].std()
+        ms[st:en] *= mult[st:en, None]
+    dist = (ms * ms).sum(1)[:, None] + (data * data).sum(1)[None, :] - 2.0 * ms @ data.T
+    index = np.argmin(dist, axis=0)
+    if return_details:
+        return dict(index=index, mult=mult, model_slices=ms, dist=dist,
+                    orientations=ref_orientations[index])
+    return ref_orientations[index]
+
+
 def gen_model_slices(ac, ref_orientations, pixel_position_reciprocal, reciprocal_extent,
                      oversampling, ac_support_size, N_pixels, override_forward_with=None,
                      eps=1e-12):
