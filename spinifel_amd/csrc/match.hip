@@ -95,6 +95,25 @@ __device__ __forceinline__ void mfma_stage(const uint8_t *sA_row0, const uint8_t
     }
 }
 
+constexpr int BM2 = 256, BN2 = 256;
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ void tile_of(int bid, int nblocks, int nta, int ntb, int &ta, int &tb)
+{
+    const int per = (nblocks + kXcds - 1) / kXcds;
+    const int xcd = bid % kXcds, local = bid / kXcds;
+    int t = xcd * per + local;
+    if (t >= nblocks) t = bid;        // uneven split: fall back to the launch order
+    constexpr int GM = 8;
+    const int in_group = GM * ntb;
+    const int group = t / in_group;
+    const int first = group * GM;
+    const int gm = min(nta - first, GM);
+    const int r = t % in_group;
+    ta = first + r % gm;
+    tb = r / gm;
+}
+
 // A: rows = orientations (slices), B: rows = images.  kbytes = N_p * bytes.
 template <int PREC, int K, int OUT>
 __global__ __launch_bounds__(256) void k_gemm(const uint8_t *__restrict__ A, int64_t lda,
@@ -286,25 +305,6 @@ __global__ __launch_bounds__(256) void k_gemm(const uint8_t *__restrict__ A, int
 // dealt to the 8 XCDs round robin, so each XCD gets a contiguous range of
 // tiles, walked 8 tile-rows at a time, and the workgroups running together
 // on one XCD share A and B panels in its L2.
-constexpr int BM2 = 256, BN2 = 256;
-constexpr int kXcds = 8;
-
-__device__ __forceinline__ void tile_of(int bid, int nblocks, int nta, int ntb, int &ta, int &tb)
-{
-    const int per = (nblocks + kXcds - 1) / kXcds;
-    const int xcd = bid % kXcds, local = bid / kXcds;
-    int t = xcd * per + local;
-    if (t >= nblocks) t = bid;        // uneven split: fall back to the launch order
-    constexpr int GM = 8;
-    const int in_group = GM * ntb;
-    const int group = t / in_group;
-    const int first = group * GM;
-    const int gm = min(nta - first, GM);
-    const int r = t % in_group;
-    ta = first + r % gm;
-    tb = r / gm;
-}
-
 template <int PREC, int K>
 __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A, int64_t lda,
                                                   int64_t NA, const uint8_t *__restrict__ B,

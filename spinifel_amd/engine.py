@@ -136,23 +136,20 @@ def tiling(det_shape, tile=(16, 16)):
     return None if o is None else Tiling(o, ty * tx, (int(ty), int(tx)))
 
 
-_ORIENT_CACHE = {}
-
-
 def _orient_order(quats):
-    """Morton order of the orientations of a device quaternion buffer, cached
-    by buffer address and size (a stale order after the buffer is rewritten
-    changes only the speed, not the slices)."""
-    key = (quats.data_ptr(), quats.shape[0], str(quats.device))
-    hit = _ORIENT_CACHE.get(key)
-    if hit is None:
-        from .quaternions import morton_order
-        perm = morton_order(quats.cpu().numpy())
-        hit = torch.as_tensor(perm.astype(np.int32)).to(quats.device)
-        if len(_ORIENT_CACHE) > 64:
-            _ORIENT_CACHE.clear()
-        _ORIENT_CACHE[key] = hit
-    return hit
+    """Morton order of the orientations of a device quaternion tensor, kept on
+    the tensor object with its version counter (an address-based cache would
+    hand a reused allocation the order of other quaternions)."""
+    hit = getattr(quats, "_mtip_orient_order", None)
+    if hit is not None and hit[0] == quats._version:
+        return hit[1]
+    from .quaternions import morton_order
+    perm = torch.as_tensor(morton_order(quats.cpu().numpy()).astype(np.int32)).to(quats.device)
+    try:
+        quats._mtip_orient_order = (quats._version, perm)
+    except AttributeError:
+        pass
+    return perm
 
 
 def auto_tiling(plan, det_shape):
