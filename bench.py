@@ -535,6 +535,9 @@ def main():
                    "form": f"{head}" + (" + top-8 FP32/FP64 rescore" if head != "fp32" else "")},
         "speedup_vs_fp32": (res[head]["images_per_s"] / res["fp32"]["images_per_s"]
                             if "fp32" in res and head != "fp32" else None),
+        "speedup_vs_fp32_slices_resident": (
+            res[head]["slices_resident"]["images_per_s"] / res["fp32"]["slices_resident"]["images_per_s"]
+            if "fp32" in res and head != "fp32" else None),
         "forms": res,
         "roofline": rl,
         "kernels": roof,
