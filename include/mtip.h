@@ -139,6 +139,60 @@ int mtip_plan_slices(mtip_plan *plan, const double *quats, int64_t N_o, const do
 int mtip_rotate_positions(const double *quats, int64_t N_o, const double *q, int64_t N_p,
                           double scale, double *H, double *K, double *L, void *stream);
 
+/* ------------------------------------------------ type-1 NUFFT (adjoint) */
+
+/*
+ * The adjoint of slicing, in three steps on a plan whose M is the output
+ * edge (M for the data term, M_ups = 2M for the convolution kernel):
+ *
+ *   out[i] = scale * support[i] * Re sum_j c_j exp(+i k(i) . x_j),
+ *   k_d(i) = i_d - M/2,  out (M, M, M) C order, axis 0 paired with x (= H)
+ *
+ * i.e. spinifel's adjoint (spinifel/autocorrelation.py:97-128 and
+ * spinifel/extern/nufft_ext.py:287-327: nufft_3d_t1 with isign = +1, times
+ * support, real part, / M^3 when scale = 1 / M^3), evaluated by the plan's
+ * ES kernel: spreading onto the fine grid, then the transform back to modes.
+ *
+ * mtip_plan_spread_begin clears the plan's FP64 spreading grid (allocating
+ * it and the pass buffers on first use: (np^3 + 3 np^2 (M/2 + 1)) * 8 bytes
+ * roughly).  Any number of spread calls then accumulate; spread_finish runs
+ * the transform and writes `out` (device FP64, M^3); the grid keeps its
+ * contents until the next begin.  Calls on one plan must not overlap.
+ */
+int mtip_plan_spread_begin(mtip_plan *plan, void *stream);
+
+/*
+ * Spread row n of `values` (device FP32 (N_o, ld_values), indexed by pixel;
+ * NULL spreads ones, as Merge's nuvect of ones, sequential/autocorrelation.py:70)
+ * at the q-points of orientation n: scale * inv(R(quats[n])) . q_j, the
+ * positions of Merge.get_non_uniform_positions
+ * (spinifel/sequential/autocorrelation.py:85-108) times mult (:146-148).
+ * quats, q, pixel_order, n_order as for mtip_plan_slices with a tile of 256
+ * points.  bad_count (nullable device int): += points outside |x| < 3 pi
+ * (skipped; spinifel/autocorrelation.py:114 asserts instead).
+ */
+int mtip_plan_spread_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
+                            int64_t N_p, const int32_t *pixel_order, int64_t n_order,
+                            double scale, const float *values, int64_t ld_values,
+                            int *bad_count, void *stream);
+
+/*
+ * Spread values (device FP32 (N), NULL = ones) at arbitrary points
+ * (x, y, z device FP64 (N), radians): nufft_3d_t1(H_, K_, L_, nuvect, ...)
+ * of spinifel/autocorrelation.py:119 / extern/nufft_ext.py:316.  Tiles of 256
+ * consecutive points share an LDS box, so neighbouring points should be
+ * adjacent (as a flattened detector is).
+ */
+int mtip_plan_spread_points(mtip_plan *plan, const double *x, const double *y, const double *z,
+                            int64_t N, const float *values, int *bad_count, void *stream);
+
+/*
+ * Transform the accumulated grid to the M^3 real output (see above);
+ * support: device FP64 (M^3) or NULL for 1.
+ */
+int mtip_plan_spread_finish(mtip_plan *plan, const double *support, double scale, double *out,
+                            void *stream);
+
 /* ---------------------------------------------------------- row utilities */
 
 /* Per-row FP64 sum and sum of squares of a float32 (rows, cols) matrix. */

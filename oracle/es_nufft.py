@@ -21,10 +21,18 @@ reference, so this follows its published exponential-of-semicircle method:
 Mode order is finufft's default CMCL: model index ``i`` along each axis is
 frequency ``k = i - M//2``.  Axis 0 pairs with x (= H).
 
+The type-1 (adjoint) half, ``nufft3d1`` / ``adjoint``, spreads with the same
+kernel on the same fine grid (C, ``oracle_es_spread3d``), FFTs with the sign
+of ``isign`` and deconvolves; ``nudft3d1`` is its brute-force ground truth.
+
 Parity status: pinned by closed forms and brute force (tests/test_oracle_nufft.py):
 the FP64 direct NUDFT below, a delta at the centre (constant), a symmetric
 pair (2 cos), points on the lattice 2 pi m / M (equals numpy's FFT), axis
-permutation, and realness for centrosymmetric models.  finufft's own output
+permutation, and realness for centrosymmetric models.  Type 1
+(tests/test_oracle_adjoint.py): the brute-force sum against numpy, against
+the type-2 brute force through the adjoint identity <A f, c> = <f, A^H c>, a
+point at the origin (constant), lattice points (numpy's inverse FFT), and the
+ES type 1 against the brute force.  finufft's own output
 bits are not reproducible here (library absent); only tolerance-level
 agreement is meaningful for an approximate transform anyway.
 """
@@ -59,6 +67,12 @@ def _load():
         lib.oracle_nudft3d2_real.argtypes = [dp, ctypes.c_int64, dp, dp, dp,
                                              ctypes.c_int64, ctypes.c_int, dp]
         lib.oracle_nudft3d2_real.restype = ctypes.c_int
+        lib.oracle_es_spread3d.argtypes = [dp, dp, dp, dp, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int, ctypes.c_double, dp]
+        lib.oracle_es_spread3d.restype = ctypes.c_int
+        lib.oracle_nudft3d1.argtypes = [dp, dp, dp, dp, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.c_int, dp]
+        lib.oracle_nudft3d1.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -222,3 +236,84 @@ def nudft3d2_numpy(x, y, z, f, isign=-1):
     e2 = np.exp(1j * isign * np.outer(np.ravel(y), k))
     e3 = np.exp(1j * isign * np.outer(np.ravel(z), k))
     return np.einsum("abc,ja,jb,jc->j", f, e1, e2, e3)
+
+
+# ------------------------------------------------------------ type 1 (adjoint)
+
+def _pts(x, y, z, c):
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+    z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(np.asarray(c).ravel(), dtype=np.complex128)
+    if not (x.size == y.size == z.size == c.size):
+        raise ValueError("x, y, z, c differ in length")
+    return x, y, z, c
+
+
+def spread(x, y, z, c, n, w, beta):
+    """b (n, n, n) complex: sum_j c_j prod_d phi(l_d - X_jd) on the periodic grid (C)."""
+    lib = _load()
+    x, y, z, c = _pts(x, y, z, c)
+    grid = np.zeros((n, n, n), dtype=np.complex128)
+    rc = lib.oracle_es_spread3d(_dptr(x), _dptr(y), _dptr(z), _dptr(c.view(np.float64)),
+                                x.size, int(n), int(w), float(beta),
+                                _dptr(grid.view(np.float64)))
+    if rc:
+        raise ValueError("oracle_es_spread3d: bad arguments")
+    return grid
+
+
+def nufft3d1(x, y, z, c, M, isign=1, eps=1e-12, upsampfac=2.0):
+    """f_k = sum_j c_j exp(isign i k.(x_j, y_j, z_j)), k_d in [-M/2, (M-1)/2],
+    complex (M, M, M), axis 0 with x.
+
+    Restates ``nufft_3d_t1(H_, K_, L_, nuvect, +1, eps, M, M, M)``
+    (spinifel/autocorrelation.py:119; finufftpy ``nufft3d1`` at
+    spinifel/extern/finufft_ext.py:59): spread with the ES kernel on the fine
+    grid, FFT with the sign of ``isign``, keep modes k mod n, divide by
+    phihat(k1) phihat(k2) phihat(k3).
+    """
+    x, y, z, c = _pts(x, y, z, c)
+    if x.size and max(np.abs(x).max(), np.abs(y).max(), np.abs(z).max()) >= 3 * np.pi:
+        raise AssertionError("NU points outside [-3pi, 3pi)")
+    w, beta = es_params(eps, upsampfac)
+    n = fine_size(M, w, upsampfac)
+    b = spread(x, y, z, c, n, w, beta)
+    B = np.fft.ifftn(b) * n ** 3 if isign > 0 else np.fft.fftn(b)
+    k = np.arange(M) - M // 2
+    idx = k % n
+    ph = es_fourier(k, n, w, beta)
+    f = B[np.ix_(idx, idx, idx)]
+    return f / (ph[:, None, None] * ph[None, :, None] * ph[None, None, :])
+
+
+def nudft3d1(x, y, z, c, M, isign=1):
+    """FP64 direct type-1 sum (brute force; the ground truth), complex (M, M, M)."""
+    lib = _load()
+    x, y, z, c = _pts(x, y, z, c)
+    out = np.zeros((M, M, M), dtype=np.complex128)
+    rc = lib.oracle_nudft3d1(_dptr(x), _dptr(y), _dptr(z), _dptr(c.view(np.float64)), x.size,
+                             int(M), int(isign), _dptr(out.view(np.float64)))
+    if rc:
+        raise ValueError("oracle_nudft3d1: bad arguments")
+    return out
+
+
+def nudft3d1_numpy(x, y, z, c, M, isign=1):
+    """Pure-numpy direct type-1 sum (tiny inputs; cross-checks the C loop)."""
+    k = np.arange(M) - M // 2
+    e1 = np.exp(1j * isign * np.outer(np.ravel(x), k))
+    e2 = np.exp(1j * isign * np.outer(np.ravel(y), k))
+    e3 = np.exp(1j * isign * np.outer(np.ravel(z), k))
+    return np.einsum("j,ja,jb,jc->abc", np.ravel(c).astype(np.complex128), e1, e2, e3)
+
+
+def adjoint(nuvect, H_, K_, L_, M, use_recip_sym=True, support=None, eps=1e-12):
+    """spinifel's ``adjoint`` (spinifel/autocorrelation.py:97-128): type 1 with
+    isign +1, times support, real part when use_recip_sym, divided by M^3."""
+    ugrid = nufft3d1(H_, K_, L_, nuvect, M, 1, eps)
+    if support is not None:
+        ugrid = ugrid * support
+    if use_recip_sym:
+        ugrid = ugrid.real
+    return ugrid / M ** 3

@@ -1,5 +1,6 @@
-"""Forward (type-2) operator and q-point generation, drop-in for the parts of
-spinifel/autocorrelation.py on the orientation-matching path.
+"""Forward (type-2) and adjoint (type-1) operators and q-point generation,
+drop-in for the parts of spinifel/autocorrelation.py on the orientation-matching
+and merge paths.
 
 ``forward`` keeps the reference signature ``forward(ugrid, H_, K_, L_,
 support, use_recip_sym, N)`` (spinifel/autocorrelation.py:32-61) with its
@@ -53,6 +54,40 @@ def forward(ugrid, H_, K_, L_, support, use_recip_sym, N, eps=None):
     if out.shape[0] != N:
         raise ValueError(f"N = {N} but {out.shape[0]} points were given")
     return out
+
+
+def adjoint(nuvect, H_, K_, L_, M, use_recip_sym=True, support=None, eps=None):
+    """Re of the type-1 NUFFT (isign +1) of ``nuvect`` at (H_, K_, L_) onto the
+    M^3 grid, times ``support``, divided by M^3 -> float64 (M, M, M).
+
+    Reference signature and checks: spinifel/autocorrelation.py:97-128 (domain
+    assert :114, support :121-122, real part :124-125, / M**3 :127).  The GPU
+    path computes the real part of the transform of real data, which is what
+    every reference caller asks for (real slices, use_recip_sym=True,
+    sequential/autocorrelation.py:158-166, :195-203); complex data or
+    ``use_recip_sym=False`` raise NotImplementedError rather than return a
+    different result.
+    """
+    H_ = np.asarray(H_, dtype=np.float64).ravel()
+    K_ = np.asarray(K_, dtype=np.float64).ravel()
+    L_ = np.asarray(L_, dtype=np.float64).ravel()
+    assert H_.shape == K_.shape == L_.shape
+    assert np.max(np.abs(np.array([H_, K_, L_]))) < 3 * np.pi
+    nuvect = np.asarray(nuvect).ravel()
+    if np.iscomplexobj(nuvect):
+        if np.any(nuvect.imag != 0):
+            raise NotImplementedError("adjoint of complex data (GPU path takes real data)")
+        nuvect = nuvect.real
+    if not use_recip_sym:
+        raise NotImplementedError("use_recip_sym=False (GPU path returns the real part)")
+    if nuvect.shape[0] != H_.shape[0]:
+        raise ValueError(f"{nuvect.shape[0]} values for {H_.shape[0]} points")
+    plan = get_plan(int(M), eps)
+    out = plan.adjoint_points(H_, K_, L_, values=nuvect.astype(np.float32),
+                              support=None if support is None else
+                              np.ascontiguousarray(np.broadcast_to(
+                                  np.asarray(support, np.float64), (int(M),) * 3)))
+    return out.cpu().numpy()
 
 
 def _rotate(orientations, pixel_position_reciprocal, scale):

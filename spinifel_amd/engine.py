@@ -274,6 +274,93 @@ class NufftPlan:
         _lib.check(rc, "mtip_plan_slices")
         return dict(s32=out, bf16=bf, row_sum=rs, row_sumsq=rq, bad=bad, N_p=N_p, ld=ld)
 
+    # ---------------------------------------------------------- type 1 (adjoint)
+
+    def _support_arg(self, support):
+        if support is None:
+            return None
+        s = torch.as_tensor(support)
+        if s.dim() == 0 or s.numel() == 1:
+            v = float(s.reshape(-1)[0])
+            return None if v == 1.0 else torch.full((self.M,) * 3, v, dtype=torch.float64,
+                                                    device=self.device)
+        s = s.to(self.device, torch.float64).contiguous()
+        if tuple(s.shape) != (self.M,) * 3 and s.numel() != self.M ** 3:
+            raise ValueError(f"support shape {tuple(s.shape)} != {(self.M,) * 3}")
+        return s
+
+    def _adjoint_finish(self, support, out_scale, out, bad, check_domain, stream):
+        sup = self._support_arg(support)
+        if out is None:
+            out = torch.empty((self.M,) * 3, dtype=torch.float64, device=self.device)
+        scale = 1.0 / self.M ** 3 if out_scale is None else float(out_scale)
+        _lib.check(self.lib.mtip_plan_spread_finish(self._h, _ptr(sup), scale, _ptr(out),
+                                                    _stream(stream)), "mtip_plan_spread_finish")
+        if check_domain and int(bad.item()) > 0:
+            raise AssertionError(f"{int(bad.item())} NU points outside [-3pi, 3pi)")
+        return out
+
+    def adjoint_slices(self, quats, q, scale, values=None, support=None, out_scale=None,
+                       order=None, out=None, stream=None, check_domain=True):
+        """Re of the type-1 transform (isign +1) of ``values`` at the rotated
+        pixel q-points -- the adjoint of ``slices``:
+
+            out = out_scale * support * Re sum_j c_j exp(+i k . x_j),  (M, M, M) float64
+
+        with x_j = scale * inv(R(quats[n])) q_p; values (N_o, >= N_p) float32
+        (row n at orientation n, indexed by pixel) or None for ones;
+        out_scale defaults to 1 / M^3 (spinifel/autocorrelation.py:128).
+        order: a Tiling of 256-point tiles or None (pixel index order).
+        """
+        quats = quats.to(self.device, torch.float64).contiguous()
+        q = q.to(self.device, torch.float64).contiguous()
+        N_o, N_p = quats.shape[0], q.shape[1]
+        if isinstance(order, Tiling):
+            if order.points != 256:
+                raise ValueError("adjoint_slices spreads tiles of 256 points")
+            order = order.order
+        if order is not None:
+            order = torch.as_tensor(order).to(self.device, torch.int32).contiguous()
+        ld = 0
+        if values is not None:
+            values = torch.as_tensor(values).to(self.device, torch.float32)
+            if values.dim() == 1:
+                values = values.reshape(N_o, -1)
+            if values.stride(-1) != 1:
+                values = values.contiguous()
+            if values.shape[0] != N_o or values.shape[1] < N_p:
+                raise ValueError(f"values shape {tuple(values.shape)} for {N_o} x {N_p} points")
+            ld = values.stride(0)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        st = _stream(stream)
+        _lib.check(self.lib.mtip_plan_spread_begin(self._h, st), "mtip_plan_spread_begin")
+        _lib.check(self.lib.mtip_plan_spread_slices(
+            self._h, _ptr(quats), N_o, _ptr(q), N_p, _ptr(order),
+            order.numel() if order is not None else 0, float(scale), _ptr(values), ld,
+            _ptr(bad), st), "mtip_plan_spread_slices")
+        return self._adjoint_finish(support, out_scale, out, bad, check_domain, stream)
+
+    def adjoint_points(self, x, y, z, values=None, support=None, out_scale=None, out=None,
+                       stream=None, check_domain=True):
+        """Re of the type-1 transform (isign +1) of values at points (x, y, z)
+        (radians) -> (M, M, M) float64, times support and out_scale (1 / M^3)."""
+        xs = [torch.as_tensor(v, dtype=torch.float64).reshape(-1).to(self.device).contiguous()
+              for v in (x, y, z)]
+        N = xs[0].numel()
+        if not (xs[1].numel() == N and xs[2].numel() == N):
+            raise ValueError("x, y, z differ in length")
+        if values is not None:
+            values = torch.as_tensor(values).reshape(-1).to(self.device, torch.float32).contiguous()
+            if values.numel() != N:
+                raise ValueError(f"{values.numel()} values for {N} points")
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        st = _stream(stream)
+        _lib.check(self.lib.mtip_plan_spread_begin(self._h, st), "mtip_plan_spread_begin")
+        _lib.check(self.lib.mtip_plan_spread_points(self._h, _ptr(xs[0]), _ptr(xs[1]), _ptr(xs[2]),
+                                                    N, _ptr(values), _ptr(bad), st),
+                   "mtip_plan_spread_points")
+        return self._adjoint_finish(support, out_scale, out, bad, check_domain, stream)
+
 
 def row_stats(x, n_cols=None, stream=None):
     """FP64 per-row sum and sum of squares of a float32 (rows, ld) tensor."""
