@@ -16,7 +16,8 @@
  *    an axis is frequency k = i - M/2 (finufft's CMCL mode order); axis 0
  *    pairs with H (x), axis 2 with L (z).
  *  - No entry point allocates device memory except mtip_plan_create /
- *    mtip_plan_set_model (the plan's fine grid and tables); workspaces are
+ *    mtip_plan_set_model (the plan's fine grid and tables) and the first
+ *    mtip_plan_spread_begin on a plan (its spreading grid); workspaces are
  *    passed in by the caller with their size queried first.
  */
 #ifndef MTIP_H
@@ -29,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MTIP_ABI_VERSION 2
+#define MTIP_ABI_VERSION 3
 
 enum {
     MTIP_OK = 0,
@@ -167,14 +168,15 @@ int mtip_plan_spread_begin(mtip_plan *plan, void *stream);
  * at the q-points of orientation n: scale * inv(R(quats[n])) . q_j, the
  * positions of Merge.get_non_uniform_positions
  * (spinifel/sequential/autocorrelation.py:85-108) times mult (:146-148).
- * quats, q, pixel_order, n_order as for mtip_plan_slices with a tile of 256
- * points.  bad_count (nullable device int): += points outside |x| < 3 pi
+ * quats, q, pixel_order, n_order, tile_points as for mtip_plan_slices (a
+ * tile's points share one LDS box: 8 x 8 tiles keep the boxes of fine
+ * grids above ~200 points per edge within LDS).  bad_count (nullable device int): += points outside |x| < 3 pi
  * (skipped; spinifel/autocorrelation.py:114 asserts instead).
  */
 int mtip_plan_spread_slices(mtip_plan *plan, const double *quats, int64_t N_o, const double *q,
                             int64_t N_p, const int32_t *pixel_order, int64_t n_order,
-                            double scale, const float *values, int64_t ld_values,
-                            int *bad_count, void *stream);
+                            int tile_points, double scale, const float *values,
+                            int64_t ld_values, int *bad_count, void *stream);
 
 /*
  * Spread values (device FP32 (N), NULL = ones) at arbitrary points

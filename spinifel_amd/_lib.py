@@ -43,8 +43,8 @@ _SIGS = {
                                 _vp, _vp, _vp, _vp, _vp]),
     "mtip_rotate_positions": (_i32, [_vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp, _vp]),
     "mtip_plan_spread_begin": (_i32, [_vp, _vp]),
-    "mtip_plan_spread_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _f64, _vp, _i64,
-                                       _vp, _vp]),
+    "mtip_plan_spread_slices": (_i32, [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _f64, _vp,
+                                       _i64, _vp, _vp]),
     "mtip_plan_spread_points": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "mtip_plan_spread_finish": (_i32, [_vp, _vp, _f64, _vp, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),

@@ -19,20 +19,19 @@
 // which folds the pads onto the periodic grid because exp(2 pi i k l / n)
 // has period n in l.
 //
-// Spreading: a workgroup takes a tile of 256 points (a detector tile of one
+// Spreading: a workgroup takes a tile of points (a detector tile of one
 // orientation, or 256 consecutive caller points), bounds the union of their
-// windows, accumulates the tile in an LDS box with FP32 LDS atomics and adds
-// the box's non-zero cells to the FP64 grid with FP64 global atomics (FP64
-// keeps the sum over all tiles exact to double rounding; each LDS cell sums
-// at most a few hundred terms).  A tile whose box exceeds kBoxCap adds its
-// taps to the FP64 grid directly.
+// windows, accumulates the tile in an LDS box and adds the box's non-zero
+// cells to the FP64 grid with FP64 global atomics (FP64 keeps the sum over
+// all tiles exact to double rounding; each LDS cell sums at most a few
+// hundred terms).  A tile whose box exceeds kBoxCap adds its taps to the
+// FP64 grid directly.
 #include <math.h>
 
 #include "nufft_shared.h"
 
 namespace {
 
-constexpr int kSpreadTP = 256;     // points per tile
 constexpr int kAdjOch = 8;         // outputs per thread of a transform pass
 
 __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, int64_t n)
@@ -42,46 +41,22 @@ __global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, int64_
     for (; i < n; i += stride) p[i] = 0.0;
 }
 
-__device__ __forceinline__ void lds_add(float *p, float v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Element e (0 or 1) of a lane's pair, all w^2 (a, b) rows, into the LDS box.
-template <int W, int S>
-__device__ __forceinline__ void spread_rows_lds(float *box, int base, int plane,
-                                                const float (&wa)[W], const float (&wb)[W],
-                                                float v)
-{
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-        const float va = wa[a] * v;
-#pragma unroll
-        for (int b = 0; b < W; ++b) lds_add(box + base + a * plane + b * S, va * wb[b]);
-    }
-}
-
-template <int W>
-__device__ __forceinline__ void spread_rows_direct(double *acc, int64_t base, int np,
-                                                   const float (&wa)[W], const float (&wb)[W],
-                                                   float v)
-{
-    const int64_t plane = (int64_t)np * np;
-#pragma unroll
-    for (int a = 0; a < W; ++a) {
-        const float va = wa[a] * v;
-#pragma unroll
-        for (int b = 0; b < W; ++b)
-            unsafeAtomicAdd(acc + base + a * plane + (int64_t)b * np, (double)(va * wb[b]));
-    }
-}
-
 // GEOM_ROTATE: workgroup o spreads row o of `vals` at the N_p pixel q-vectors
 // rotated by inv(R(quats[o])) and scaled by gscale (grid units), tile after
-// tile in the caller's pixel order.  GEOM_POINTS: workgroup b spreads points
-// [256 b, 256 b + 256) of (px, py, pz) (radians; gscale = n / 2 pi).
-// vals == nullptr spreads ones.
-template <int W>
+// tile of TP points in the caller's pixel order.  GEOM_POINTS: workgroup b
+// spreads points [TP b, TP b + TP) of (px, py, pz) (radians; gscale =
+// n / 2 pi).  vals == nullptr spreads ones.
+//
+// Per tile: the TP window corners bound a box of the fine grid; when it fits
+// kBoxCap the workgroup accumulates the tile there with plain LDS
+// read-add-writes and then adds the box's non-zero cells to the FP64 grid.
+// No LDS atomics (they run about one lane per clock on this part): wave v
+// owns the box planes whose local x index is v mod 4, and each wave walks all
+// of the tile's points in order, its lanes covering distinct (y, z) cells of
+// the window (lane = row * W + element), so no two lanes ever update one cell
+// at once and a wave's LDS accesses complete in issue order.  A box beyond
+// kBoxCap sends the same per-lane taps to the FP64 grid as global atomics.
+template <int W, int TP>
 __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restrict__ acc, int geom,
                                                  const double *__restrict__ quats,
                                                  const double *__restrict__ q, int64_t N_p,
@@ -92,20 +67,17 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
                                                  double gscale, const float *__restrict__ vals,
                                                  int64_t ld_v, int *bad)
 {
-    using PT = PairTaps<W>;
-    constexpr int GP = PT::GP, NS = PT::NS, PPW = 64 / GP;
-    constexpr int TP = kSpreadTP, PW = TP / 4, STEPS = PW / PPW;
-    __shared__ __align__(16) float s_box[kBoxCap];
-    // per point: pair base (box-local when staged, padded grid otherwise) with
-    // the window parity in bit 0, or -1 for no contribution
-    __shared__ int s_lb[TP];
-    __shared__ float s_v[TP];
-    __shared__ float s_t[3][TP];
+    constexpr int RP = W * W <= 64 ? W : 64 / W;     // window rows per lane pass
+    constexpr int NPASS = (W + RP - 1) / RP;
+    __shared__ float s_box[kBoxCap];
+    __shared__ int s_base[TP];          // window corner: box-local or padded-grid index, -1 none
+    __shared__ int s_xc[TP];            // box-local x of the corner
+    __shared__ float4 s_pt[TP];         // fractional offsets tx, ty, tz and the value
     __shared__ int s_red_i[4][6];
     __shared__ double s_ri[9];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int g = lane / GP, c = lane % GP;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t o = blockIdx.x;
     const int np = gv.np;
     if (geom == GEOM_ROTATE) {
@@ -117,17 +89,20 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
         }
         __syncthreads();
     }
+    // this lane's window row and element (rows pass * RP + lrow)
+    const int lrow = lane / W, lel = lane % W;
+    const bool lane_on = lane < RP * W;
     int nbad = 0;
     const int64_t n_slots =
         geom == GEOM_POINTS ? TP : (order ? n_order : mtip::ceil_div(N_p, TP) * TP);
 
     for (int64_t tb = 0; tb < n_slots; tb += TP) {
-        // ---- this thread's point
+        // ---- this thread's point (threads < TP)
         bool ok = false;
         float v = 0.f;
         int ijk[3] = {0, 0, 0};
         float t[3] = {0.f, 0.f, 0.f};
-        {
+        if (tid < TP) {
             const int64_t k = tb + tid;
             double X0 = 0.0, X1 = 0.0, X2 = 0.0;
             bool have = false;
@@ -176,88 +151,73 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
             hi[e] = -min(min(s_red_i[0][e + 3], s_red_i[1][e + 3]),
                          min(s_red_i[2][e + 3], s_red_i[3][e + 3]));
         }
-        const bool any = lo[0] != 0x7fffffff;
-        if (!any) {
+        if (lo[0] == 0x7fffffff) {
             __syncthreads();     // s_red_i is rewritten by the next tile
             continue;
         }
-        const int lz = lo[2] & ~1;
-        const int bx = hi[0] - lo[0] + W, by = hi[1] - lo[1] + W;
-        const int bz = hi[2] - lz + 2 * GP;
-        const int S = pair_stride(bz);
-        const bool staged = S > 0 && bx * by * S <= kBoxCap;
-        int lb = -1;
-        if (ok) {
-            const int zb = ijk[2] & ~1;
-            lb = staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * S + (zb - lz)
-                        : (ijk[0] * np + ijk[1]) * np + zb;
-            lb |= ijk[2] & 1;
+        const int bx = hi[0] - lo[0] + W, by = hi[1] - lo[1] + W, bz = hi[2] - lo[2] + W;
+        const bool staged = bx * by * bz <= kBoxCap;
+        const int ncell = staged ? bx * by * bz : 0;
+        if (tid < TP) {
+            s_base[tid] = !ok ? -1
+                        : staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * bz + (ijk[2] - lo[2])
+                                 : (ijk[0] * np + ijk[1]) * np + ijk[2];
+            s_xc[tid] = ijk[0] - lo[0];
+            s_pt[tid] = make_float4(t[0], t[1], t[2], v);
         }
-        s_lb[tid] = lb;
-        s_v[tid] = v;
-        s_t[0][tid] = t[0];
-        s_t[1][tid] = t[1];
-        s_t[2][tid] = t[2];
-        const int ncell = staged ? bx * by * S : 0;
-        for (int k = tid; 2 * k < ncell; k += 256)
-            reinterpret_cast<float2 *>(s_box)[k] = make_float2(0.f, 0.f);
+        for (int k = tid; k < ncell; k += 256) s_box[k] = 0.f;
         __syncthreads();
-        // ---- taps: each wave its PW points, PPW per step; the points of one
-        // step are STEPS slots apart so that fewer of them share a window
-        // corner (equal addresses serialise LDS atomics)
-        const int plane = by * S;
+        // ---- taps: every wave walks all TP points, updating its own x planes;
+        // a chunk of 64 points' parameters is read into registers at once and
+        // each point's taken from its lane by readlane (scalar operands)
+        const int plane = staged ? by * bz : np * np, rowstride = staged ? bz : np;
 #pragma unroll 1
-        for (int it = 0; it < STEPS; ++it) {
-            const int pl = wave * PW + g * STEPS + it;
-            const int lv = s_lb[pl];
-            const float tx = s_t[0][pl], ty = s_t[1][pl], tz = s_t[2][pl], pv = s_v[pl];
-            const int sh = lv & 1;
-            float own_a[NS], own_b[NS];
+        for (int c0 = 0; c0 < TP; c0 += 64) {
+            const int cl = c0 + lane < TP ? c0 + lane : TP - 1;
+            const int r_base = s_base[cl], r_xc = s_xc[cl];
+            const float4 r_pt = s_pt[cl];
+            const int nc = TP - c0 < 64 ? TP - c0 : 64;
+            for (int i = 0; i < nc; ++i) {
+                const int base = __builtin_amdgcn_readlane(r_base, i);
+                if (base < 0) continue;
+                const float tx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.x), i));
+                const float ty = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.y), i));
+                const float tz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.z), i));
+                const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.w), i));
+                const int a0 = (wave - __builtin_amdgcn_readlane(r_xc, i)) & 3;
+                // lanes 0 .. W-1 hold the x weights, read back per plane
+                const float pa = es_phi((float)lane - tx, gv.beta_l2e, gv.c2);
+                const float wz = es_phi((float)lel - tz, gv.beta_l2e, gv.c2) * pv;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                own_a[s] = es_phi((float)(c + GP * s) - tx, gv.beta_l2e, gv.c2);
-                own_b[s] = es_phi((float)(c + GP * s) - ty, gv.beta_l2e, gv.c2);
-            }
-            float wa[W], wb[W];
-            gather_weights<GP, W>(own_a, wa, lane);
-            gather_weights<GP, W>(own_b, wb, lane);
-            if (lv < 0) continue;
-            // elements 2c - sh and 2c + 1 - sh of the window
-            const int k0 = 2 * c - sh;
-            const int base = (lv & ~1) + 2 * c;
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int ke = k0 + e;
-                if (ke < 0 || ke >= W) continue;
-                const float ve = pv * es_phi((float)ke - tz, gv.beta_l2e, gv.c2);
-                if (staged) {
-                    switch (S) {
-#define MTIP_SPREAD_CASE(SS) \
-    case SS: spread_rows_lds<W, SS>(s_box, base + e, plane, wa, wb, ve); break;
-                        MTIP_PAIR_STRIDES(MTIP_SPREAD_CASE)
-#undef MTIP_SPREAD_CASE
-                    default: break;
+                for (int ps = 0; ps < NPASS; ++ps) {
+                    const int b = ps * RP + lrow;
+                    const float wyz = es_phi((float)b - ty, gv.beta_l2e, gv.c2) * wz;
+                    const int off = base + b * rowstride + lel;
+                    if (lane_on && b < W) {
+                        for (int a = a0; a < W; a += 4) {
+                            const float wa =
+                                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa), a));
+                            if (staged) s_box[off + a * plane] += wa * wyz;
+                            else unsafeAtomicAdd(acc + (int64_t)off + (int64_t)a * plane,
+                                                 (double)(wa * wyz));
+                        }
                     }
-                } else {
-                    spread_rows_direct<W>(acc, (int64_t)base + e, np, wa, wb, ve);
                 }
             }
         }
         __syncthreads();
         // ---- the box's non-zero cells into the FP64 grid
         if (staged) {
-            const float inv_S = 1.0f / (float)S, inv_by = 1.0f / (float)by;
+            const float inv_bz = 1.0f / (float)bz, inv_by = 1.0f / (float)by;
             for (int k = tid; k < ncell; k += 256) {
                 const float val = s_box[k];
                 if (val != 0.f) {
-                    // k / S and r / by for k < 2^16: the float estimates are
+                    // k / bz and r / by for k < 2^16: the float estimates are
                     // within half a unit of the integer quotient
-                    const int r = (int)(((float)k + 0.5f) * inv_S), z = k - r * S;
+                    const int r = (int)(((float)k + 0.5f) * inv_bz), z = k - r * bz;
                     const int xr = (int)(((float)r + 0.5f) * inv_by), yr = r - xr * by;
-                    const int gz = lz + z;
-                    if (gz < np)
-                        unsafeAtomicAdd(acc + ((int64_t)(lo[0] + xr) * np + (lo[1] + yr)) * np + gz,
-                                        (double)val);
+                    unsafeAtomicAdd(acc + ((int64_t)(lo[0] + xr) * np + (lo[1] + yr)) * np + lo[2] + z,
+                                    (double)val);
                 }
             }
         }
@@ -361,13 +321,19 @@ __global__ __launch_bounds__(256) void k_adj_pass(const double *__restrict__ in_
 
 template <int W>
 void launch_spread(const GridView &gv, double *acc, int geom, const double *quats, int64_t N_o,
-                   const double *q, int64_t N_p, const int32_t *order, int64_t n_order,
+                   const double *q, int64_t N_p, const int32_t *order, int64_t n_order, int tile,
                    const double *x, const double *y, const double *z, int64_t N, double gscale,
                    const float *vals, int64_t ld, int *bad, hipStream_t st)
 {
-    const int64_t nb = geom == GEOM_ROTATE ? N_o : mtip::ceil_div(N, kSpreadTP);
-    hipLaunchKernelGGL((k_spread<W>), dim3((unsigned)nb), dim3(256), 0, st, gv, acc, geom, quats,
-                       q, N_p, order, n_order, x, y, z, N, gscale, vals, ld, bad);
+#define MTIP_SPREAD_TP(T)                                                                      \
+    hipLaunchKernelGGL((k_spread<W, T>),                                                       \
+                       dim3((unsigned)(geom == GEOM_ROTATE ? N_o : mtip::ceil_div(N, T))),     \
+                       dim3(256), 0, st, gv, acc, geom, quats, q, N_p, order, n_order, x, y, z, \
+                       N, gscale, vals, ld, bad)
+    if (tile == 64) MTIP_SPREAD_TP(64);
+    else if (tile == 128) MTIP_SPREAD_TP(128);
+    else MTIP_SPREAD_TP(256);
+#undef MTIP_SPREAD_TP
 }
 
 }  // namespace
@@ -405,7 +371,8 @@ extern "C" int mtip_plan_spread_begin(mtip_plan *p, void *stream)
 
 extern "C" int mtip_plan_spread_slices(mtip_plan *p, const double *quats, int64_t N_o,
                                        const double *q, int64_t N_p, const int32_t *pixel_order,
-                                       int64_t n_order, double scale, const float *values,
+                                       int64_t n_order, int tile_points, double scale,
+                                       const float *values,
                                        int64_t ld_values, int *bad_count, void *stream)
 {
     MTIP_REQUIRE(p, MTIP_ERR_ARG, "mtip_plan_spread_slices: null plan");
@@ -419,13 +386,17 @@ extern "C" int mtip_plan_spread_slices(mtip_plan *p, const double *quats, int64_
     MTIP_REQUIRE(quats && q, MTIP_ERR_ARG, "mtip_plan_spread_slices: null pointer");
     MTIP_REQUIRE(!values || ld_values >= N_p, MTIP_ERR_ARG,
                  "mtip_plan_spread_slices: ld_values < N_p");
-    MTIP_REQUIRE(!pixel_order || (n_order > 0 && n_order % kSpreadTP == 0), MTIP_ERR_ARG,
-                 "mtip_plan_spread_slices: n_order must be a positive multiple of 256");
+    if (tile_points == 0) tile_points = 256;
+    MTIP_REQUIRE(tile_points == 64 || tile_points == 128 || tile_points == 256, MTIP_ERR_ARG,
+                 "mtip_plan_spread_slices: tile_points must be 0, 64, 128 or 256");
+    MTIP_REQUIRE(!pixel_order || (n_order > 0 && n_order % tile_points == 0), MTIP_ERR_ARG,
+                 "mtip_plan_spread_slices: n_order must be a positive multiple of tile_points");
     DeviceGuard dg(p->device);
     GridView gv = view_of(p);
     const double gscale = scale * (double)p->n / (2.0 * M_PI);
     MTIP_W_DISPATCH(p->w, launch_spread, gv, p->acc, (int)GEOM_ROTATE, quats, N_o, q, N_p,
-                    pixel_order, n_order, nullptr, nullptr, nullptr, (int64_t)0, gscale, values,
+                    pixel_order, n_order, tile_points, nullptr, nullptr, nullptr, (int64_t)0,
+                    gscale, values,
                     ld_values, bad_count, (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
@@ -441,13 +412,13 @@ extern "C" int mtip_plan_spread_points(mtip_plan *p, const double *x, const doub
     MTIP_REQUIRE(N >= 0, MTIP_ERR_ARG, "mtip_plan_spread_points: N < 0");
     if (N == 0) return MTIP_OK;
     MTIP_REQUIRE(x && y && z, MTIP_ERR_ARG, "mtip_plan_spread_points: null pointer");
-    MTIP_REQUIRE(mtip::ceil_div(N, kSpreadTP) <= 0x7fffffff, MTIP_ERR_ARG,
+    MTIP_REQUIRE(mtip::ceil_div(N, 256) <= 0x7fffffff, MTIP_ERR_ARG,
                  "mtip_plan_spread_points: N too large");
     DeviceGuard dg(p->device);
     GridView gv = view_of(p);
     const double gscale = (double)p->n / (2.0 * M_PI);
     MTIP_W_DISPATCH(p->w, launch_spread, gv, p->acc, (int)GEOM_POINTS, nullptr, (int64_t)0,
-                    nullptr, (int64_t)0, nullptr, (int64_t)0, x, y, z, N, gscale, values,
+                    nullptr, (int64_t)0, nullptr, (int64_t)0, 256, x, y, z, N, gscale, values,
                     (int64_t)0, bad_count, (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;

@@ -310,15 +310,15 @@ class NufftPlan:
         with x_j = scale * inv(R(quats[n])) q_p; values (N_o, >= N_p) float32
         (row n at orientation n, indexed by pixel) or None for ones;
         out_scale defaults to 1 / M^3 (spinifel/autocorrelation.py:128).
-        order: a Tiling of 256-point tiles or None (pixel index order).
+        order: a Tiling (its tiles share one LDS box; ``auto_tiling`` picks
+        16 x 16 or 8 x 8 by grid size) or None (pixel index order, 256 at a time).
         """
         quats = quats.to(self.device, torch.float64).contiguous()
         q = q.to(self.device, torch.float64).contiguous()
         N_o, N_p = quats.shape[0], q.shape[1]
+        tile_points = 256
         if isinstance(order, Tiling):
-            if order.points != 256:
-                raise ValueError("adjoint_slices spreads tiles of 256 points")
-            order = order.order
+            order, tile_points = order.order, order.points
         if order is not None:
             order = torch.as_tensor(order).to(self.device, torch.int32).contiguous()
         ld = 0
@@ -336,7 +336,7 @@ class NufftPlan:
         _lib.check(self.lib.mtip_plan_spread_begin(self._h, st), "mtip_plan_spread_begin")
         _lib.check(self.lib.mtip_plan_spread_slices(
             self._h, _ptr(quats), N_o, _ptr(q), N_p, _ptr(order),
-            order.numel() if order is not None else 0, float(scale), _ptr(values), ld,
+            order.numel() if order is not None else 0, tile_points, float(scale), _ptr(values), ld,
             _ptr(bad), st), "mtip_plan_spread_slices")
         return self._adjoint_finish(support, out_scale, out, bad, check_domain, stream)
 

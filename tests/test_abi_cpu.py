@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
 
 def test_abi_version_and_workspace_size():
     lib = _lib.load()
-    assert lib.mtip_abi_version() == 2
+    assert lib.mtip_abi_version() == 3
     assert lib.mtip_match_workspace_size(10000, 10000, 8, 1) == 10000 * 79 * 8 * 8
     assert lib.mtip_match_workspace_size(1, 1, 1, 0) == 8
     assert lib.mtip_match_workspace_size(0, 5, 1, 0) == 0

@@ -54,7 +54,8 @@ def test_points_vs_brute_force(torch_dev, M):
     assert rel_l2(out, ref) < TOL
 
 
-@pytest.mark.parametrize("M,tile", [(41, (16, 16)), (41, None), (82, (16, 16))])
+@pytest.mark.parametrize("M,tile", [(41, (16, 16)), (41, (8, 16)), (41, None), (82, (16, 16)),
+                                    (82, (8, 8))])
 def test_slices_vs_oracle(torch_dev, M, tile):
     """Detector tiles of one orientation per workgroup (LDS box path); M = 82
     is the upsampled edge of the convolution kernel for M = 41 (values = ones,
@@ -143,7 +144,7 @@ def test_spread_calls_accumulate(torch_dev):
     _lib.check(lib.mtip_plan_spread_begin(plan._h, st))
     for lo, hi in ((0, 25), (25, 40)):
         _lib.check(lib.mtip_plan_spread_slices(
-            plan._h, ctypes.c_void_p(qt[lo:].data_ptr()), hi - lo, p(q), N_p, None, 0,
+            plan._h, ctypes.c_void_p(qt[lo:].data_ptr()), hi - lo, p(q), N_p, None, 0, 0,
             float(np.pi / E), ctypes.c_void_p(vals[lo:].data_ptr()), N_p, p(bad), st))
     _lib.check(lib.mtip_plan_spread_finish(plan._h, p(sup), 2.0, p(out), st))
     one = plan.adjoint_slices(qt, q, np.pi / E, values=vals, support=sup, out_scale=2.0)
