@@ -106,6 +106,31 @@ class NUFFT:
             raise ValueError(f"N = {N} but {en - st} orientations x {self.N_pixels} pixels")
         return out
 
+    def adjoint(self, nuvect, H_, K_, L_, support, use_reciprocal_symmetry, M):
+        """extern/nufft_ext.py:287-327: type 1 (isign +1) of real ``nuvect`` at
+        (H_, K_, L_) onto M^3, times support, real part, / M^3 -> float64
+        (M, M, M) device tensor.  Complex data or use_reciprocal_symmetry=False
+        raise NotImplementedError (the GPU path returns the real part)."""
+        if not use_reciprocal_symmetry:
+            raise NotImplementedError("use_reciprocal_symmetry=False (GPU path returns the real part)")
+        if torch.is_tensor(nuvect):
+            if nuvect.is_complex():
+                if bool((nuvect.imag != 0).any()):
+                    raise NotImplementedError("adjoint of complex data")
+                nuvect = nuvect.real
+        else:
+            nuvect = np.asarray(nuvect)
+            if np.iscomplexobj(nuvect):
+                if np.any(nuvect.imag != 0):
+                    raise NotImplementedError("adjoint of complex data")
+                nuvect = nuvect.real
+        sup = None
+        if support is not None and not (np.isscalar(support) and support == 1):
+            sup = support if torch.is_tensor(support) else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(support, np.float64), (int(M),) * 3))
+        return self.plan(int(M)).adjoint_points(H_, K_, L_, values=torch.as_tensor(nuvect).reshape(-1),
+                                                support=sup)
+
 
 class SNM:
     """Slicing and matching with the data images resident

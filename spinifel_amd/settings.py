@@ -4,7 +4,9 @@ orientation-matching path reads.
 Reference: ``SpinifelSettings`` (spinifel/settings.py:167-848); the path
 reads ``N_orientations``, ``N_batch_size``, ``reduced_det_shape``
 (spinifel/sequential/orientation_matching.py:204-211) and ``oversampling``
-(:234-236).  The rest of the reference's TOML/CLI system is out of scope.
+(:234-236), and the merge reads ``M``, ``M_ups``, ``solve_ac_maxiter``
+(spinifel/sequential/autocorrelation.py:44-53).  The rest of the reference's
+TOML/CLI system is out of scope.
 
 Fields added for this implementation: ``nufft_eps`` (the reference hard-codes
 1e-12 for finufft, spinifel/autocorrelation.py:59; FP32 slices cannot carry
@@ -22,6 +24,7 @@ class SpinifelSettings:
     reduced_det_shape: tuple = (1, 128, 128)
     oversampling: int = 1
     nufft_eps: float = 1e-6
+    solve_ac_maxiter: int = 100     # CG iterations of the merge (settings.py:383-386)
     precision: str = "fp32"     # 'fp32' | 'bf16' | 'fp8'
     topk: int = 0               # 0: 1 for fp32, 8 otherwise
     rescore: bool = None        # None: False for fp32, True otherwise
@@ -37,6 +40,11 @@ class SpinifelSettings:
     def M(self):
         """4 * Mquat + 1 (spinifel/settings.py:813-818)."""
         return 4 * self.Mquat + 1
+
+    @property
+    def M_ups(self):
+        """2 * M, the grid of the merge's convolution kernel (spinifel/settings.py:821-825)."""
+        return 2 * self.M
 
 
 settings = SpinifelSettings()
