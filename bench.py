@@ -60,6 +60,11 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="rule-based tiles, no autotuning")
     p.add_argument("--no-batching", action="store_true", help="skip the batching form")
     p.add_argument("--no-serving", action="store_true", help="skip the serving form")
+    p.add_argument("--no-merge", action="store_true", help="skip the merge (type-1 + CG) leg")
+    p.add_argument("--merge-only", action="store_true",
+                   help="run only the merge leg (for profiling its kernels)")
+    p.add_argument("--merge-images", type=int, default=1000,
+                   help="images of the merge leg (x 16,384 px)")
     p.add_argument("--batch-steps", type=int, default=5, help="timed graph replays per model")
     p.add_argument("--serve-seconds", type=float, default=8.0,
                    help="length of the Poisson arrival schedule")
@@ -435,6 +440,120 @@ def run_serving(args, torch, engine, synth, log):
     return out
 
 
+def run_merge(args, torch, engine, synth, log):
+    """The merge step after matching: type-1 spreading of 1000 images x 16,384 px
+    at M = 81 (data term) and M_ups = 162 (convolution kernel, ones), then
+    Merge.solve_ac with 100 CG iterations; plus the oracle's C spreading on a
+    sample as the CPU baseline."""
+    from spinifel_amd import _lib
+    from spinifel_amd.merge import Merge
+    from spinifel_amd.settings import SpinifelSettings
+    from spinifel_amd.quaternions import random_quaternions
+    dev = torch.device("cuda", 0)
+    ppr, pdr = synth.detector(1, 128, 128)
+    E = float(pdr.max())
+    q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
+    N_p = q.shape[1]
+    n_img = int(args.merge_images)
+    settings = SpinifelSettings(reduced_det_shape=(1, 128, 128), nufft_eps=args.eps)
+    M = settings.M
+    ac = synth.make_model(M, MODELS[M], args.seed)
+    plan_m = engine.NufftPlan(M, args.eps, 0)
+    plan_m.set_model(ac)
+    quats_np = random_quaternions(n_img, synth.rng_for(args.seed, 21))
+    quats = torch.as_tensor(quats_np).to(dev)
+    images = gpu_images(plan_m, quats_np, q, np.pi / E, engine.auto_tiling(plan_m, (128, 128)),
+                        1.0, args.seed, torch)
+    mg = Merge(settings, images, ppr, pdr)
+    lib = _lib.load()
+    st = engine._stream()
+    P = engine._ptr
+    npts = n_img * N_p
+    out = dict(workload=f"M={M} (M_ups={settings.M_ups}), {n_img} images x 1x128x128 px "
+                        f"= {npts / 1e6:.1f}M points, eps {args.eps}, CG maxiter "
+                        f"{settings.solve_ac_maxiter}")
+    # ---- spreading kernel and transform passes (HIP events on the launch stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    for name, plan, vals, tiling in (("data_M81", mg._plan, mg._data, mg._tiling),
+                                     ("ones_M162", mg._plan_ups, None, mg._tiling_ups)):
+        ords = torch.as_tensor(tiling.order).to(dev)
+        res_buf = torch.empty((plan.M,) * 3, dtype=torch.float64, device=dev)
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        t_sp, t_fi = [], []
+        for r in range(args.warmup + args.steps):
+            _lib.check(lib.mtip_plan_spread_begin(plan._h, st))
+            ev[0].record()
+            _lib.check(lib.mtip_plan_spread_slices(
+                plan._h, P(quats), n_img, P(q), N_p, P(ords), ords.numel(), tiling.points,
+                float(mg.mult), P(vals), N_p if vals is not None else 0, P(bad), st))
+            ev[1].record()
+            _lib.check(lib.mtip_plan_spread_finish(plan._h, None, 1.0 / plan.M ** 3, P(res_buf),
+                                                   st))
+            ev[2].record()
+            torch.cuda.synchronize()
+            if r >= args.warmup:
+                t_sp.append(ev[0].elapsed_time(ev[1]))
+                t_fi.append(ev[1].elapsed_time(ev[2]))
+        sp, fi = float(np.mean(t_sp)), float(np.mean(t_fi))
+        npad = plan.padded_edge
+        # algorithmic HBM bytes of one spread launch: the values (4 B per point;
+        # none for ones) and one FP64 write of the padded grid
+        alg = (4 * npts if vals is not None else 0) + 8 * npad ** 3
+        lds = 2 * 4 * npts * plan.width ** 3        # read + write of every tap in LDS
+        out[name] = dict(spread_ms=sp, finish_ms=fi, tile=list(tiling.shape),
+                         points_per_s=npts / (sp * 1e-3),
+                         hbm_alg_bytes=alg, hbm_alg_gbps=alg / (sp * 1e-3) / 1e9,
+                         hbm_frac=alg / (sp * 1e-3) / 8e12,
+                         lds_tap_bytes=lds, lds_tap_tbps=lds / (sp * 1e-3) / 1e12,
+                         lds_frac=lds / (sp * 1e-3) / (256 * 128 * 2.4e9),
+                         bad=int(bad.item()))
+        log("merge", name, json.dumps(out[name]))
+    # ---- whole solve_ac (positions on the fly, two type-1 transforms, CG)
+    t_all, its = [], []
+    for r in range(max(1, args.warmup // 2) + 2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mg.solve_ac(0, quats_np)
+        torch.cuda.synchronize()
+        if r >= max(1, args.warmup // 2):
+            t_all.append(time.perf_counter() - t0)
+            its.append(mg.last["iterations"])
+    # one CG operator application (both FFT convolutions) timed alone
+    x = mg.last["ac"].reshape(-1).clone()
+    Wop = mg.last["W"]
+    for _ in range(3):
+        Wop(x)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(10):
+        Wop(x)
+    ev[1].record()
+    torch.cuda.synchronize()
+    out["solve_ac"] = dict(ms=1e3 * float(np.mean(t_all)), iterations=int(its[-1]),
+                           info=int(mg.last["info"]),
+                           W_apply_ms=ev[0].elapsed_time(ev[1]) / 10)
+    log("merge solve_ac", json.dumps(out["solve_ac"]))
+    # ---- CPU baseline: the oracle's C spreading (OpenMP slabs) + numpy FFT
+    # transform, eps as the GPU, on a sample of images
+    if not args.no_cpu:
+        from oracle import es_nufft, geometry
+        k = 8
+        H, K, L = geometry.gen_nonuniform_normalized_positions(quats_np[:k], ppr, E, 1)
+        vals = images[:k].double().cpu().numpy().reshape(-1)
+        t0 = time.perf_counter()
+        es_nufft.adjoint(vals, H, K, L, M, True, None, args.eps)
+        tc = time.perf_counter() - t0
+        cores = len(os.sched_getaffinity(0))
+        threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+        out["cpu_baseline"] = dict(
+            value=k * N_p / tc, unit="points/s", cores=min(cores, threads), kind="port",
+            sample=f"oracle type 1 (C spreading + numpy FFT, eps {args.eps}) of {k} images "
+                   f"x {N_p} px at M={M} ({tc:.2f} s)")
+        log("merge cpu", json.dumps(out["cpu_baseline"]))
+    del mg, images
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -454,6 +573,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n}), flush=True)
+        return
+    if args.merge_only:
+        print(json.dumps({"merge": run_merge(args, torch, engine, synth, log)}), flush=True)
         return
     forms = [f for f in args.forms.split(",") if f]
     log("building workload", args.workload)
@@ -513,6 +635,8 @@ def main():
     batching = None if args.no_batching else run_batching(args, torch, engine, synth, log)
     torch.cuda.empty_cache()
     serving_res = None if args.no_serving else run_serving(args, torch, engine, synth, log)
+    torch.cuda.empty_cache()
+    merge_res = None if args.no_merge else run_merge(args, torch, engine, synth, log)
     p = W["p"]
     line = {
         "metric": "images/s oriented, 10k orients x 16k px: BF16+rescore vs FP32; p50/p99 latency",
@@ -548,6 +672,7 @@ def main():
                    "gemm_timings": gemm_tune},
         "batching": batching,
         "serving": serving_res,
+        "merge": merge_res,
     }
     print(json.dumps(line), flush=True)
 

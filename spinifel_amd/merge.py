@@ -159,6 +159,8 @@ class Merge:
         self._plan_ups = get_plan(self.M_ups, self.eps, self.device.index or 0)
         self._tiling = engine.auto_tiling(self._plan, self.det_shape)
         self._tiling_ups = engine.auto_tiling(self._plan_ups, self.det_shape)
+        # zero-padded operand of the convolution; only [:M]^3 is ever written
+        self._ups = torch.zeros((self.M_ups,) * 3, dtype=torch.float64, device=self.device)
 
         def callback(xk):
             callback.counter += 1
@@ -174,24 +176,31 @@ class Merge:
         itself forms them inside the spreading kernel."""
         return gen_nonuniform_positions(orientations, self.pixel_position_reciprocal)
 
+    # The reference wraps both per-iteration transforms in ifftshift / fftshift
+    # of the real-space grid.  A product in frequency space is a circular
+    # convolution, which commutes with cyclic shifts, and fftshift undoes
+    # ifftshift, so fftshift(ifft(fft(ifftshift(x)) S)) = ifft(fft(x) S):
+    # the two rolls per transform are dropped (the kernel's own ifftshift,
+    # applied once in setup_linops, stays).
+
     def fourier_reg(self, uvect, support):
         """:110-122 -- support, FFT, antisupport mask, inverse FFT, support."""
         M = self.M
         g = uvect.view(M, M, M) * support
-        X = torch.fft.rfftn(torch.fft.ifftshift(g))
+        X = torch.fft.rfftn(g)
         X *= self._reg_mask
-        reg = torch.fft.fftshift(torch.fft.irfftn(X, s=(M, M, M)))
+        reg = torch.fft.irfftn(X, s=(M, M, M))
         return (reg * support).reshape(-1)
 
     def core_problem_convolution(self, uvect, F_ugrid_conv_half, ac_support):
         """:124-145 -- zero-pad to M_ups at [:M], multiply by the kernel's
         spectrum, crop [:M], support."""
         M, Mu = self.M, self.M_ups
-        ups = torch.zeros((Mu,) * 3, dtype=torch.float64, device=self.device)
+        ups = self._ups
         ups[:M, :M, :M] = uvect.view(M, M, M) * ac_support
-        X = torch.fft.rfftn(torch.fft.ifftshift(ups))
+        X = torch.fft.rfftn(ups)
         X *= F_ugrid_conv_half
-        out = torch.fft.fftshift(torch.fft.irfftn(X, s=(Mu,) * 3))[:M, :M, :M] * ac_support
+        out = torch.fft.irfftn(X, s=(Mu,) * 3)[:M, :M, :M] * ac_support
         return out.reshape(-1)
 
     def _quats(self, orientations):
