@@ -195,6 +195,28 @@ int mtip_plan_spread_points(mtip_plan *plan, const double *x, const double *y, c
 int mtip_plan_spread_finish(mtip_plan *plan, const double *support, double scale, double *out,
                             void *stream);
 
+/* ---------------------------------------------------------------- phasing */
+
+/*
+ * The two projections of each ER / HIO iteration of spinifel's phasing
+ * (spinifel/sequential/phasing.py:129-194), FP64, for a real density whose
+ * spectrum the caller keeps as the half spectrum of a real-to-complex
+ * transform (Hermitian, so the replaced spectrum's inverse is the real part
+ * the reference keeps).
+ *
+ * mtip_phase_amplitudes: spectrum (device complex128, n entries, in place)
+ * <- amplitudes * exp(i angle(spectrum)) everywhere but entry 0, the masked
+ * zero frequency (step_phase, :187-190, with amp_mask_ of :269-270).
+ *
+ * mtip_phase_update: rho (device FP64, n, in place) from the projected
+ * density rho_mod and the support (device uint8): ER (mode 0, :141-146) or
+ * HIO with feedback beta (mode 1, :162-167), with the rho_max clamp
+ * (+inf as phase() passes, :279).
+ */
+int mtip_phase_amplitudes(void *spectrum, const double *amplitudes, int64_t n, void *stream);
+int mtip_phase_update(double *rho, const double *rho_mod, const uint8_t *support, int64_t n,
+                      int mode, double beta, double rho_max, void *stream);
+
 /* ---------------------------------------------------------- row utilities */
 
 /* Per-row FP64 sum and sum of squares of a float32 (rows, cols) matrix. */

@@ -38,7 +38,7 @@ Differences a caller can observe:
 import numpy as np
 import torch
 
-from . import engine
+from . import engine, filters
 from .autocorrelation import get_plan, gen_nonuniform_positions
 from .quaternions import random_quaternions
 
@@ -56,29 +56,10 @@ def antisupport(M, oversampling):
     return F
 
 
-def _reflect_index(n, radius, device):
-    i = np.arange(-radius, n + radius)
-    i = np.mod(i, 2 * n)
-    i = np.where(i >= n, 2 * n - 1 - i, i)
-    return torch.as_tensor(i, dtype=torch.long, device=device)
-
-
 def gaussian_filter(a, sigma=0.5, truncate=4.0):
     """scipy.ndimage.gaussian_filter(a, sigma) on a device tensor (FP64,
     separable, mode 'reflect'; spinifel/sequential/autocorrelation.py:201)."""
-    radius = int(truncate * float(sigma) + 0.5)
-    x = np.arange(-radius, radius + 1, dtype=np.float64)
-    w = np.exp(-0.5 * (x / sigma) ** 2)
-    w /= w.sum()
-    out = a.to(torch.float64)
-    for ax in range(out.dim()):
-        n = out.shape[ax]
-        pad = out.index_select(ax, _reflect_index(n, radius, out.device))
-        acc = torch.zeros_like(out)
-        for t in range(2 * radius + 1):
-            acc += float(w[t]) * pad.narrow(ax, t, n)
-        out = acc
-    return out
+    return filters.gaussian_filter(a, sigma, "reflect", truncate)
 
 
 def cg(matvec, b, x0, maxiter, tol=1e-5, callback=None):

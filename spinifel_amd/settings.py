@@ -5,8 +5,9 @@ Reference: ``SpinifelSettings`` (spinifel/settings.py:167-848); the path
 reads ``N_orientations``, ``N_batch_size``, ``reduced_det_shape``
 (spinifel/sequential/orientation_matching.py:204-211) and ``oversampling``
 (:234-236), and the merge reads ``M``, ``M_ups``, ``solve_ac_maxiter``
-(spinifel/sequential/autocorrelation.py:44-53).  The rest of the reference's
-TOML/CLI system is out of scope.
+(spinifel/sequential/autocorrelation.py:44-53), and phasing reads ``Mquat``,
+``beta``, ``nER``, ``nHIO``, ``N_phase_loops`` (spinifel/sequential/phasing.py:249-296).
+The rest of the reference's TOML/CLI system is out of scope.
 
 Fields added for this implementation: ``nufft_eps`` (the reference hard-codes
 1e-12 for finufft, spinifel/autocorrelation.py:59; FP32 slices cannot carry
@@ -25,6 +26,10 @@ class SpinifelSettings:
     oversampling: int = 1
     nufft_eps: float = 1e-6
     solve_ac_maxiter: int = 100     # CG iterations of the merge (settings.py:383-386)
+    beta: float = 0.9               # HIO feedback (settings.py:388-391)
+    nER: int = 10                   # ER iterations per block (settings.py:398-401)
+    nHIO: int = 5                   # HIO iterations per block (settings.py:403-406)
+    N_phase_loops: int = 5          # ER/HIO/ER/shrink-wrap loops (settings.py:408-411)
     precision: str = "fp32"     # 'fp32' | 'bf16' | 'fp8'
     topk: int = 0               # 0: 1 for fp32, 8 otherwise
     rescore: bool = None        # None: False for fp32, True otherwise
