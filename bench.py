@@ -60,7 +60,8 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="rule-based tiles, no autotuning")
     p.add_argument("--no-batching", action="store_true", help="skip the batching form")
     p.add_argument("--no-serving", action="store_true", help="skip the serving form")
-    p.add_argument("--no-merge", action="store_true", help="skip the merge (type-1 + CG) leg")
+    p.add_argument("--no-merge", action="store_true",
+                   help="skip the merge (type-1 + CG) and phasing leg")
     p.add_argument("--merge-only", action="store_true",
                    help="run only the merge leg (for profiling its kernels)")
     p.add_argument("--merge-images", type=int, default=1000,
@@ -551,6 +552,49 @@ def run_merge(args, torch, engine, synth, log):
                    f"x {N_p} px at M={M} ({tc:.2f} s)")
         log("merge cpu", json.dumps(out["cpu_baseline"]))
     del mg, images
+    # ---- phasing of the matching model's autocorrelation (default settings:
+    # 5 loops of 10 ER / 5 HIO / 10 ER + shrink-wrap, then 10 ER)
+    from spinifel_amd import phasing
+    ev2 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    np.random.seed(args.seed)
+    phasing.phase(0, ac, settings=settings)
+    torch.cuda.synchronize()
+    t_ph = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        phasing.phase(0, ac, settings=settings)
+        torch.cuda.synchronize()
+        t_ph.append(time.perf_counter() - t0)
+    n_it = settings.N_phase_loops * (2 * settings.nER + settings.nHIO) + settings.nER
+    Mt = settings.M
+    rho = torch.rand((Mt,) * 3, dtype=torch.float64, device=dev)
+    amp = torch.rand((Mt, Mt, Mt // 2 + 1), dtype=torch.float64, device=dev)
+    ph = phasing._Phaser(amp, rho > 0.5, rho, Mt)
+    for _ in range(3):
+        ph.step(1, 0.9)
+    ev2[0].record()
+    for _ in range(20):
+        ph.step(1, 0.9)
+    ev2[1].record()
+    torch.cuda.synchronize()
+    out["phasing"] = dict(ms=1e3 * float(np.mean(t_ph)), iterations=n_it,
+                          iteration_ms=ev2[0].elapsed_time(ev2[1]) / 20, M=Mt)
+    if not args.no_cpu:
+        # the oracle (numpy FP64 complex FFTs, as the reference's CPU path) on
+        # 5 ER iterations, scaled to the full schedule
+        from oracle import phasing as oph
+        ac_filt_, amp_, mask = oph.amplitudes_from_ac(ac)
+        sup = oph.create_support_(ac_filt_, Mt, settings.Mquat)
+        r0 = sup * np.random.default_rng(args.seed).uniform(size=(Mt,) * 3)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            oph.ER(r0, amp_, mask, sup, np.inf)
+        tc = (time.perf_counter() - t0) / 5
+        out["phasing"]["cpu_baseline"] = dict(
+            value=1e3 * tc * n_it, unit="ms per phase()", kind="port", cores=1,
+            sample=f"oracle ER iteration at M={Mt} (numpy FP64 FFTs), 5 iterations "
+                   f"({1e3 * tc:.1f} ms each) x {n_it}")
+    log("phasing", json.dumps(out["phasing"]))
     return out
 
 
