@@ -20,6 +20,7 @@ error against FP64 distances (numpy) on an image subset; NUFFT slice error
 against the FP64 direct NUDFT on a pixel subsample.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -45,7 +46,7 @@ def p_n_img(W):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="m81_16k_px")
     p.add_argument("--forms", default="fp32,bf16,fp8")
@@ -130,13 +131,20 @@ def run_form(form, W, args, torch, engine, timers):
     torch.cuda.synchronize()
     timers.clear()
     lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        sl, r = step(timers)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - s0)
-    total = time.perf_counter() - t0
+    # the interpreter's cyclic collector runs between steps, not inside them
+    # (a collection inside a step stalls the launches and the GPU idles)
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s0 = time.perf_counter()
+            sl, r = step(timers)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - s0)
+        total = time.perf_counter() - t0
+    finally:
+        gc.enable()
     N_i = imgs.N_i
     out = dict(images_per_s=N_i * args.steps / total, ms_per_step=1e3 * total / args.steps,
                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),

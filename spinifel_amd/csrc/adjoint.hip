@@ -69,7 +69,14 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
 {
     constexpr int RP = W * W <= 64 ? W : 64 / W;     // window rows per lane pass
     constexpr int NPASS = (W + RP - 1) / RP;
-    __shared__ float s_box[kBoxCap];
+    // small tiles keep each point's 3 W kernel weights (the z ones times the
+    // value) in LDS, computed once per tile instead of by every wave; their
+    // room comes out of the box so three workgroups still fit a CU
+    constexpr bool PRE = TP <= 128;
+    constexpr int NW = 3 * W;
+    constexpr int CAP = PRE ? kBoxCap - TP * NW : kBoxCap;
+    __shared__ float s_box[CAP];
+    __shared__ float s_w[PRE ? TP * NW : 1];
     __shared__ int s_base[TP];          // window corner: box-local or padded-grid index, -1 none
     __shared__ int s_xc[TP];            // box-local x of the corner
     __shared__ float4 s_pt[TP];         // fractional offsets tx, ty, tz and the value
@@ -156,7 +163,7 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
             continue;
         }
         const int bx = hi[0] - lo[0] + W, by = hi[1] - lo[1] + W, bz = hi[2] - lo[2] + W;
-        const bool staged = bx * by * bz <= kBoxCap;
+        const bool staged = bx * by * bz <= CAP;
         const int ncell = staged ? bx * by * bz : 0;
         if (tid < TP) {
             s_base[tid] = !ok ? -1
@@ -167,41 +174,101 @@ __global__ __launch_bounds__(256, 3) void k_spread(GridView gv, double *__restri
         }
         for (int k = tid; k < ncell; k += 256) s_box[k] = 0.f;
         __syncthreads();
+        if constexpr (PRE) {
+            for (int k = tid; k < TP * NW; k += 256) {
+                const int pn = k / NW, r = k - pn * NW, d = r / W, e = r - d * W;
+                const float4 pt = s_pt[pn];
+                const float tt = d == 0 ? pt.x : d == 1 ? pt.y : pt.z;
+                const float w = es_phi((float)e - tt, gv.beta_l2e, gv.c2);
+                s_w[k] = d == 2 ? w * pt.w : w;
+            }
+            __syncthreads();
+        }
         // ---- taps: every wave walks all TP points, updating its own x planes;
         // a chunk of 64 points' parameters is read into registers at once and
-        // each point's taken from its lane by readlane (scalar operands)
+        // each point's taken from its lane by readlane (scalar operands).  A
+        // point's planes (a0, a0 + 4, ...) are read together, then added and
+        // written, so one LDS round trip serves them all; with precomputed
+        // weights the next point's are read while this point's planes wait.
         const int plane = staged ? by * bz : np * np, rowstride = staged ? bz : np;
+        constexpr int NPL = (W + 3) / 4;      // planes per point and wave, at most
+        auto taps = [&](int base, int a0, float pa, const float (&wyz)[NPASS])
+            __attribute__((always_inline)) {
+#pragma unroll
+            for (int ps = 0; ps < NPASS; ++ps) {
+                const int b = ps * RP + lrow;
+                if (!lane_on || b >= W) continue;
+                const int off = base + b * rowstride + lel + a0 * plane;
+                float wa[NPL];
+#pragma unroll
+                for (int j = 0; j < NPL; ++j)
+                    wa[j] = a0 + 4 * j < W ? __int_as_float(__builtin_amdgcn_readlane(
+                                                 __float_as_int(pa), a0 + 4 * j))
+                                           : 0.f;
+                if (staged) {
+                    float cur[NPL];
+#pragma unroll
+                    for (int j = 0; j < NPL; ++j)
+                        if (a0 + 4 * j < W) cur[j] = s_box[off + 4 * j * plane];
+#pragma unroll
+                    for (int j = 0; j < NPL; ++j)
+                        if (a0 + 4 * j < W) s_box[off + 4 * j * plane] = fmaf(wa[j], wyz[ps], cur[j]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < NPL; ++j)
+                        if (a0 + 4 * j < W)
+                            unsafeAtomicAdd(acc + (int64_t)off + (int64_t)(4 * j) * plane,
+                                            (double)(wa[j] * wyz[ps]));
+                }
+            }
+        };
 #pragma unroll 1
         for (int c0 = 0; c0 < TP; c0 += 64) {
             const int cl = c0 + lane < TP ? c0 + lane : TP - 1;
             const int r_base = s_base[cl], r_xc = s_xc[cl];
-            const float4 r_pt = s_pt[cl];
             const int nc = TP - c0 < 64 ? TP - c0 : 64;
-            for (int i = 0; i < nc; ++i) {
-                const int base = __builtin_amdgcn_readlane(r_base, i);
-                if (base < 0) continue;
-                const float tx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.x), i));
-                const float ty = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.y), i));
-                const float tz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.z), i));
-                const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.w), i));
-                const int a0 = (wave - __builtin_amdgcn_readlane(r_xc, i)) & 3;
-                // lanes 0 .. W-1 hold the x weights, read back per plane
-                const float pa = es_phi((float)lane - tx, gv.beta_l2e, gv.c2);
-                const float wz = es_phi((float)lel - tz, gv.beta_l2e, gv.c2) * pv;
+            if constexpr (PRE) {
+                // weights of point i: x by lane (lanes < W), y by row, z (x value) by element
+                auto load = [&](int i, float &pa, float (&wy)[NPASS], float &wz)
+                    __attribute__((always_inline)) {
+                    const float *wp = s_w + (c0 + i) * NW;
+                    pa = wp[lane < W ? lane : 0];
+                    wz = wp[2 * W + lel];
 #pragma unroll
-                for (int ps = 0; ps < NPASS; ++ps) {
-                    const int b = ps * RP + lrow;
-                    const float wyz = es_phi((float)b - ty, gv.beta_l2e, gv.c2) * wz;
-                    const int off = base + b * rowstride + lel;
-                    if (lane_on && b < W) {
-                        for (int a = a0; a < W; a += 4) {
-                            const float wa =
-                                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa), a));
-                            if (staged) s_box[off + a * plane] += wa * wyz;
-                            else unsafeAtomicAdd(acc + (int64_t)off + (int64_t)a * plane,
-                                                 (double)(wa * wyz));
-                        }
+                    for (int ps = 0; ps < NPASS; ++ps) {
+                        const int b = ps * RP + lrow;
+                        wy[ps] = wp[W + (b < W ? b : W - 1)];
                     }
+                };
+                float n_pa, n_wy[NPASS], n_wz;
+                load(0, n_pa, n_wy, n_wz);
+                for (int i = 0; i < nc; ++i) {
+                    const float pa = n_pa, wz = n_wz;
+                    float wyz[NPASS];
+#pragma unroll
+                    for (int ps = 0; ps < NPASS; ++ps) wyz[ps] = n_wy[ps] * wz;
+                    if (i + 1 < nc) load(i + 1, n_pa, n_wy, n_wz);
+                    const int base = __builtin_amdgcn_readlane(r_base, i);
+                    if (base < 0) continue;
+                    taps(base, (wave - __builtin_amdgcn_readlane(r_xc, i)) & 3, pa, wyz);
+                }
+            } else {
+                const float4 r_pt = s_pt[cl];
+                for (int i = 0; i < nc; ++i) {
+                    const int base = __builtin_amdgcn_readlane(r_base, i);
+                    if (base < 0) continue;
+                    const float tx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.x), i));
+                    const float ty = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.y), i));
+                    const float tz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.z), i));
+                    const float pv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r_pt.w), i));
+                    // lanes 0 .. W-1 hold the x weights, read back per plane
+                    const float pa = es_phi((float)lane - tx, gv.beta_l2e, gv.c2);
+                    const float wz = es_phi((float)lel - tz, gv.beta_l2e, gv.c2) * pv;
+                    float wyz[NPASS];
+#pragma unroll
+                    for (int ps = 0; ps < NPASS; ++ps)
+                        wyz[ps] = es_phi((float)(ps * RP + lrow) - ty, gv.beta_l2e, gv.c2) * wz;
+                    taps(base, (wave - __builtin_amdgcn_readlane(r_xc, i)) & 3, pa, wyz);
                 }
             }
         }

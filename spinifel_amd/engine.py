@@ -161,6 +161,16 @@ def auto_tiling(plan, det_shape):
     return tiling(det_shape, (16, 16) if plan.n_fine <= 200 else (8, 8))
 
 
+def auto_spread_tiling(plan, det_shape):
+    """Tiling for the type-1 spreading, by rule: 8 x 16 blocks while their
+    LDS box fits (fine grids up to ~200 points per edge), 8 x 8 beyond
+    (measured at 16.4M points: 8.7 ms for 8 x 16 vs 11.0 for 16 x 16 at
+    n = 162; 12.0 ms for 8 x 8 vs 59 for 8 x 16 at n = 324)."""
+    if len(tuple(det_shape)) < 2:
+        return None
+    return tiling(det_shape, (8, 16) if plan.n_fine <= 200 else (8, 8))
+
+
 class NufftPlan:
     """Type-2 NUFFT plan of a cubic M^3 model (ES kernel, sigma = 2)."""
 

@@ -138,8 +138,8 @@ class Merge:
         self.nufft = nufft
         self._plan = get_plan(self.M, self.eps, self.device.index or 0)
         self._plan_ups = get_plan(self.M_ups, self.eps, self.device.index or 0)
-        self._tiling = engine.auto_tiling(self._plan, self.det_shape)
-        self._tiling_ups = engine.auto_tiling(self._plan_ups, self.det_shape)
+        self._tiling = engine.auto_spread_tiling(self._plan, self.det_shape)
+        self._tiling_ups = engine.auto_spread_tiling(self._plan_ups, self.det_shape)
         # zero-padded operand of the convolution; only [:M]^3 is ever written
         self._ups = torch.zeros((self.M_ups,) * 3, dtype=torch.float64, device=self.device)
 
