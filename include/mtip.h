@@ -217,6 +217,26 @@ int mtip_phase_amplitudes(void *spectrum, const double *amplitudes, int64_t n, v
 int mtip_phase_update(double *rho, const double *rho_mod, const uint8_t *support, int64_t n,
                       int mode, double beta, double rho_max, void *stream);
 
+/* ------------------------------------------------------------- data prep */
+
+/*
+ * Clipping to the low-resolution region followed by 2^n_bin x 2^n_bin
+ * binning, in one pass: spinifel/prep.py's binning_sum / binning_mean /
+ * binning_index = bin2nx2n_{sum,mean,index}(clipping(arr, N_clipping),
+ * N_binning) (prep.py:36-134), with the reductions in the reference's order
+ * and type.  in: device (n_lead, panels, sa, sb) C order of dtype 0 float32,
+ * 1 float64, 2 int32, 3 int64; out: device (n_lead, panels, sa >> (n_clip +
+ * n_bin), sb >> (n_clip + n_bin)) of the same type.  mode 0 sum, 1 mean
+ * (float only), 2 index (min of each 2 x 2 block, floor-halved, per level).
+ * Clipping keeps the centred window with 1 panel and the bottom corners with
+ * 4 panels (prep.py:86-100); other panel counts give zeros, as the
+ * reference's array stays zero.  The clipped panel edges must be multiples of
+ * 2^n_bin (n_bin <= 4).  clipping_index's subtraction of the minimum
+ * (:108-113) is left to the caller.
+ */
+int mtip_clip_bin(const void *in, int dtype, int64_t n_lead, int panels, int sa, int sb,
+                  int n_clip, int n_bin, int mode, void *out, void *stream);
+
 /* ---------------------------------------------------------- row utilities */
 
 /* Per-row FP64 sum and sum of squares of a float32 (rows, cols) matrix. */

@@ -49,6 +49,7 @@ _SIGS = {
     "mtip_plan_spread_finish": (_i32, [_vp, _vp, _f64, _vp, _vp]),
     "mtip_phase_amplitudes": (_i32, [_vp, _vp, _i64, _vp]),
     "mtip_phase_update": (_i32, [_vp, _vp, _vp, _i64, _i32, _f64, _f64, _vp]),
+    "mtip_clip_bin": (_i32, [_vp, _i32, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "mtip_scaling_ratio": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     "mtip_segment_scaling_ratio": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),

@@ -6,7 +6,8 @@ reads ``N_orientations``, ``N_batch_size``, ``reduced_det_shape``
 (spinifel/sequential/orientation_matching.py:204-211) and ``oversampling``
 (:234-236), and the merge reads ``M``, ``M_ups``, ``solve_ac_maxiter``
 (spinifel/sequential/autocorrelation.py:44-53), and phasing reads ``Mquat``,
-``beta``, ``nER``, ``nHIO``, ``N_phase_loops`` (spinifel/sequential/phasing.py:249-296).
+``beta``, ``nER``, ``nHIO``, ``N_phase_loops`` (spinifel/sequential/phasing.py:249-296),
+and the data prep ``N_clipping``, ``N_binning`` (spinifel/prep.py:116-134).
 The rest of the reference's TOML/CLI system is out of scope.
 
 Fields added for this implementation: ``nufft_eps`` (the reference hard-codes
@@ -30,6 +31,8 @@ class SpinifelSettings:
     nER: int = 10                   # ER iterations per block (settings.py:398-401)
     nHIO: int = 5                   # HIO iterations per block (settings.py:403-406)
     N_phase_loops: int = 5          # ER/HIO/ER/shrink-wrap loops (settings.py:408-411)
+    N_clipping: int = 1             # data prep clipping factor 2^n (settings.py:413-416)
+    N_binning: int = 4              # data prep binning factor 2^n (settings.py:418-421)
     precision: str = "fp32"     # 'fp32' | 'bf16' | 'fp8'
     topk: int = 0               # 0: 1 for fp32, 8 otherwise
     rescore: bool = None        # None: False for fp32, True otherwise
