@@ -73,3 +73,43 @@ def test_product_path_fails_loudly_without_gpu():
         orientation_matching.slicing_and_match(
             np.ones((5, 5, 5), np.float32), np.ones((2, 1, 4, 4), np.float32),
             np.ones((3, 1, 4, 4)), np.ones((1, 4, 4)), np.eye(4), settings=s)
+
+
+def test_argument_errors_of_adjoint_phasing_and_prep_entry_points():
+    """The type-1, phasing and prep entry points reject bad arguments before
+    any device work."""
+    lib = _lib.load()
+    nul = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.mtip_plan_spread_begin(nul, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_plan_spread_slices(nul, fake, 1, fake, 1, nul, 0, 0, 1.0, nul, 0, nul,
+                                       nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_plan_spread_points(nul, fake, fake, fake, 4, nul, nul, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_plan_spread_finish(nul, nul, 1.0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_phase_update(fake, fake, fake, 8, 2, 0.9, 1.0, nul) == _lib.MTIP_ERR_ARG
+    assert "mode" in _lib.last_error()
+    assert lib.mtip_phase_update(fake, fake, fake, -1, 0, 0.9, 1.0, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_phase_amplitudes(nul, fake, 8, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_phase_amplitudes(nul, nul, 0, nul) == _lib.MTIP_OK     # nothing to do
+    # clip + bin: dtype, mode, mean of integers, edge multiples, null pointers
+    assert lib.mtip_clip_bin(fake, 7, 1, 1, 8, 8, 0, 1, 0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_clip_bin(fake, 0, 1, 1, 8, 8, 0, 1, 3, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_clip_bin(fake, 2, 1, 1, 8, 8, 0, 1, 1, fake, nul) == _lib.MTIP_ERR_ARG
+    assert "float" in _lib.last_error()
+    assert lib.mtip_clip_bin(fake, 0, 1, 1, 12, 12, 0, 3, 0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_clip_bin(fake, 0, 1, 1, 8, 8, 0, 5, 0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_clip_bin(nul, 0, 1, 1, 8, 8, 0, 1, 0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_clip_bin(nul, 0, 0, 1, 8, 8, 0, 1, 0, nul, nul) == _lib.MTIP_OK
+
+
+def test_prep_loaders_name_h5py():
+    """The HDF5 loaders keep the reference's logic and say what is missing."""
+    pytest.importorskip("torch")
+    try:
+        import h5py  # noqa: F401
+        pytest.skip("h5py is installed")
+    except ImportError:
+        pass
+    from spinifel_amd import prep
+    with pytest.raises(ImportError, match="h5py"):
+        prep.load_slices(np.zeros((1, 1, 4, 4)), 0, 1, "/nonexistent.h5")
