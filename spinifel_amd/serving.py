@@ -82,6 +82,10 @@ class Server:
         self._lock = threading.Condition()
         self._streams = [torch.cuda.Stream(device=self.device) for _ in range(n_slots)]
         self._busy = [None] * n_slots          # (requests, event, outputs) per slot
+        # per slot a pinned host staging buffer, reused while the slot is free:
+        # requests are copied into it once and uploaded from it asynchronously
+        self._stage = [dict() for _ in range(n_slots + 1)]   # per N_p; the last serves match_now
+        self._now_lock = threading.Lock()
         self._next_id = 0
         self.error = None
         self._stop = False
@@ -110,6 +114,8 @@ class Server:
             raise AssertionError("NU points outside [-3pi, 3pi)")
         op = sl.operand(self.precision, True)
         self.models[name] = ServedModel(name, np.asarray(ref_quats), sl, op, sl.N_p, ppr.shape[1:])
+        for slot in range(self.n_slots + 1):     # pinned staging allocated up front
+            self._stage_buffer(slot, self.max_batch_images, sl.N_p)
         return self.models[name]
 
     # ------------------------------------------------------------- requests
@@ -131,23 +137,44 @@ class Server:
         """Synchronous single-request match (no queue), the 'alone' reference."""
         r = Request(-1, model, np.ascontiguousarray(images, np.float32).reshape(len(images), -1),
                     time.perf_counter())
-        out = self._launch([r], torch.cuda.current_stream(self.device))
-        torch.cuda.synchronize(self.device)
-        return self._collect([r], out)[0]
+        with self._now_lock:
+            out = self._launch([r], torch.cuda.current_stream(self.device), self.n_slots)
+            torch.cuda.synchronize(self.device)
+            return self._collect([r], out)[0]
 
     # ------------------------------------------------------------- engine
     def _batch_bytes(self, n_img, N_p):
         per = N_p * (4 + 4 + 2) + 64 * self.K
         return n_img * per + (64 << 20)
 
-    def _launch(self, reqs, stream):
+    def _stage_buffer(self, slot, n, n_p):
+        buf = self._stage[slot].get(n_p)
+        if buf is None or buf.shape[0] < n:
+            buf = torch.empty((max(self.max_batch_images, n), n_p), dtype=torch.float32,
+                              pin_memory=True)
+            self._stage[slot][n_p] = buf
+        return buf
+
+    def _staged(self, reqs, slot, n, n_p):
+        """The batch's images in the slot's pinned buffer (n, n_p) float32.  The
+        buffer is only rewritten once the slot's previous batch has retired,
+        i.e. after its upload completed."""
+        buf = self._stage_buffer(slot, n, n_p)
+        a = 0
+        for r in reqs:
+            k = len(r.images)
+            # torch's copy runs on its intra-op threads, outside the interpreter lock
+            buf[a:a + k].copy_(torch.from_numpy(r.images))
+            a += k
+        return buf[:n]
+
+    def _launch(self, reqs, stream, slot=0):
         m = self.models[reqs[0].model]
         sizes = [len(r.images) for r in reqs]
         offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        host = np.concatenate([r.images for r in reqs]) if len(reqs) > 1 else reqs[0].images
+        host = self._staged(reqs, slot, int(offs[-1]), m.N_p)
         with torch.cuda.stream(stream):
-            imgs = engine.ImageSet(torch.from_numpy(host).pin_memory().to(self.device, non_blocking=True),
-                                   self.device, stream)
+            imgs = engine.ImageSet(host.to(self.device, non_blocking=True), self.device, stream)
             alpha = engine.segment_scaling_ratio(imgs.row_sum, imgs.row_sumsq, offs,
                                                  m.slices.row_sum, m.slices.row_sumsq, m.N_p, stream)
             D = imgs.operand(self.precision, alpha, m.slices.center, stream)
@@ -249,7 +276,7 @@ class Server:
                         t = time.perf_counter()
                         for r in batch:
                             r.t_start = t
-                        self._busy[s] = (batch, self._launch(batch, self._streams[s]))
+                        self._busy[s] = (batch, self._launch(batch, self._streams[s], s))
                         progressed = True
                 if not progressed:
                     self._lock.wait(timeout=0.0005)
