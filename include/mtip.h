@@ -75,8 +75,9 @@ int mtip_plan_info(const mtip_plan *plan, int *n_fine, int *width, double *beta,
 /*
  * Load a model: symmetrise (Re of the transform = transform of
  * (a(k) + a(-k)) / 2), deconvolve by the kernel's Fourier series, and
- * evaluate the real padded fine grid by separable FP64 DFTs; the grid stays
- * resident in the plan.  ac: device float32 M^3.
+ * evaluate the real padded fine grid by one FP64 n^3 FFT (hipFFT; the first
+ * call on a plan allocates its buffer and plan); the grid stays resident in
+ * the plan.  ac: device float32 M^3.
  * Replaces the ugrid upload and the deconvolve/FFT inside
  * plan.execute(nuvect, ugrid) (spinifel/extern/nufft_ext.py:283,
  * cufinufft_ext.py:344) and autocorrelation.forward's `ugrid *= support`

@@ -52,7 +52,8 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-L/opt/rocm/lib",
+           "-lhipfft", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
     return OUT

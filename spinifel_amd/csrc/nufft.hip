@@ -6,11 +6,14 @@
 //   b_l    = sum_k g_k exp(-2 pi i k . l / n),  g_k = sym(a)_k / prod phihat(k_d)
 //   Re c_j = sum_l b_l prod_d phi(X_jd - l_d),  X = x n / (2 pi)
 // sym(a)_k = (a(k) + a(-k)) / 2 makes b real (the reference keeps only the
-// real part of the transform), so the fine grid is a real float32 array of
+// real part of the transform); b comes from one FP64 hipFFT transform of the
+// n^3 grid, and the fine grid is a real float32 array of
 // (n + 2 pad)^3 values, indices l in [-n/2 - pad, n/2 + pad), evaluated
 // directly at the padded indices so no wrap is needed while interpolating.
 #include <math.h>
 #include <vector>
+
+#include <hipfft/hipfft.h>
 
 #include "nufft_shared.h"
 
@@ -83,83 +86,44 @@ __global__ void k_twiddle(double2 *tw, int n)
     }
 }
 
-constexpr int kOch = 8;    // outputs per thread along the transformed axis
-
-// One separable DFT pass: out[a][o][c] = sum_k in(a, k, c) * tw((k * l(o)) mod n),
-// k in [-h, h], l(o) = o - off.  MODE 0 reads the model (axis 2, C = 1) with
-// symmetrisation + deconvolution; MODE 1 is complex -> complex; MODE 2
-// writes the real part as float.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_dft_pass(const float *__restrict__ ac,
-                                                  const double2 *__restrict__ in,
-                                                  double2 *__restrict__ out_c,
-                                                  float *__restrict__ out_r,
-                                                  const double *__restrict__ dinv,
-                                                  const double2 *__restrict__ tw, int M, int h,
-                                                  int n, int np, int off, int64_t A, int64_t C)
+// Fine grid by FFT: G holds g_k = sym(a)_k / prod phihat(k_d) at index
+// k mod n (zero elsewhere), one thread per G entry so no clearing pass is
+// needed; hipFFT's forward Z2Z gives b_l = sum_k g_k exp(-2 pi i k . l / n),
+// real because g is real and even; k_pad_grid writes b at the padded indices
+// (periodic in l) as float32.
+__global__ __launch_bounds__(256) void k_scatter_model(const float *__restrict__ ac,
+                                                       const double *__restrict__ dinv,
+                                                       double2 *__restrict__ G, int M, int h,
+                                                       int n)
 {
-    extern __shared__ double2 s_tw[];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_tw[i] = tw[i];
-    __syncthreads();
-
-    const int K = 2 * h + 1;
-    const int nchunk = (np + kOch - 1) / kOch;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A * nchunk * C) return;
-    int64_t c = t % C;
-    int64_t r = t / C;
-    int chunk = (int)(r % nchunk);
-    int64_t a = r / nchunk;
-    int o0 = chunk * kOch;
-
-    int lmod[kOch];
-#pragma unroll
-    for (int j = 0; j < kOch; ++j) {
-        int l = o0 + j - off;
-        lmod[j] = ((l % n) + n) % n;
+    const int64_t n3 = (int64_t)n * n * n;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const int i3 = (int)(i % n), i2 = (int)((i / n) % n), i1 = (int)(i / ((int64_t)n * n));
+    const int k1 = i1 <= n / 2 ? i1 : i1 - n, k2 = i2 <= n / 2 ? i2 : i2 - n,
+              k3 = i3 <= n / 2 ? i3 : i3 - n;
+    double v = 0.0;
+    if (abs(k1) <= h && abs(k2) <= h && abs(k3) <= h) {
+        const int lo = -(M / 2), hi = (M - 1) / 2;
+        double s = 0.0;
+        if (k1 >= lo && k1 <= hi && k2 >= lo && k2 <= hi && k3 >= lo && k3 <= hi)
+            s += ac[((int64_t)(k1 + M / 2) * M + (k2 + M / 2)) * M + (k3 + M / 2)];
+        if (-k1 >= lo && -k1 <= hi && -k2 >= lo && -k2 <= hi && -k3 >= lo && -k3 <= hi)
+            s += ac[((int64_t)(-k1 + M / 2) * M + (-k2 + M / 2)) * M + (-k3 + M / 2)];
+        v = 0.5 * s * dinv[k1 + h] * dinv[k2 + h] * dinv[k3 + h];
     }
-    double acc_re[kOch], acc_im[kOch];
-#pragma unroll
-    for (int j = 0; j < kOch; ++j) acc_re[j] = acc_im[j] = 0.0;
+    G[i] = make_double2(v, 0.0);
+}
 
-    for (int kk = 0; kk < K; ++kk) {
-        int k = kk - h;
-        double vr, vi;
-        if (MODE == 0) {
-            // a = k1 * K + k2 ; transformed axis is k3
-            int k1 = (int)(a / K) - h, k2 = (int)(a % K) - h, k3 = k;
-            int lo = -(M / 2), hi = (M - 1) / 2;
-            double s = 0.0;
-            if (k1 >= lo && k1 <= hi && k2 >= lo && k2 <= hi && k3 >= lo && k3 <= hi)
-                s += ac[((int64_t)(k1 + M / 2) * M + (k2 + M / 2)) * M + (k3 + M / 2)];
-            if (-k1 >= lo && -k1 <= hi && -k2 >= lo && -k2 <= hi && -k3 >= lo && -k3 <= hi)
-                s += ac[((int64_t)(-k1 + M / 2) * M + (-k2 + M / 2)) * M + (-k3 + M / 2)];
-            vr = 0.5 * s * dinv[k1 + h] * dinv[k2 + h] * dinv[k3 + h];
-            vi = 0.0;
-        } else {
-            double2 v = in[(a * K + kk) * C + c];
-            vr = v.x;
-            vi = v.y;
-        }
-        int kmod = ((k % n) + n) % n;
-#pragma unroll
-        for (int j = 0; j < kOch; ++j) {
-            double2 e = s_tw[(kmod * lmod[j]) % n];
-            acc_re[j] += vr * e.x - vi * e.y;
-            acc_im[j] += vr * e.y + vi * e.x;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kOch; ++j) {
-        int o = o0 + j;
-        if (o < np) {
-            int64_t idx = (a * np + o) * C + c;
-            if (MODE == 2)
-                out_r[idx] = (float)acc_re[j];
-            else
-                out_c[idx] = make_double2(acc_re[j], acc_im[j]);
-        }
-    }
+__global__ __launch_bounds__(256) void k_pad_grid(const double2 *__restrict__ G,
+                                                  float *__restrict__ grid, int n, int np, int off)
+{
+    const int64_t np3 = (int64_t)np * np * np;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np3) return;
+    const int p3 = (int)(i % np), p2 = (int)((i / np) % np), p1 = (int)(i / ((int64_t)np * np));
+    auto wrap = [n, off](int p) { return ((p - off) % n + n) % n; };
+    grid[i] = (float)G[((int64_t)wrap(p1) * n + wrap(p2)) * n + wrap(p3)].x;
 }
 
 // ------------------------------------------------------------ interpolation
@@ -748,8 +712,6 @@ extern "C" int mtip_plan_create(int M, double eps, int device, mtip_plan **out)
     hipError_t err = hipMalloc(&p->grid, ng * sizeof(float));
     if (err == hipSuccess) err = hipMalloc(&p->dinv, p->K * sizeof(double));
     if (err == hipSuccess) err = hipMalloc(&p->tw, p->n * sizeof(double2));
-    if (err == hipSuccess) err = hipMalloc(&p->t1, (size_t)p->K * p->K * p->np * sizeof(double2));
-    if (err == hipSuccess) err = hipMalloc(&p->t2, (size_t)p->K * p->np * p->np * sizeof(double2));
     if (err == hipSuccess)
         err = hipMemcpy(p->dinv, dinv.data(), p->K * sizeof(double), hipMemcpyHostToDevice);
     if (err == hipSuccess) {
@@ -762,6 +724,7 @@ extern "C" int mtip_plan_create(int M, double eps, int device, mtip_plan **out)
         mtip_plan_destroy(p);
         return mtip::fail(err == hipErrorOutOfMemory ? MTIP_ERR_NOMEM : MTIP_ERR_HIP, msg);
     }
+
     *out = p;
     return MTIP_OK;
 }
@@ -773,8 +736,8 @@ extern "C" int mtip_plan_destroy(mtip_plan *p)
     (void)hipFree(p->grid);
     (void)hipFree(p->dinv);
     (void)hipFree(p->tw);
-    (void)hipFree(p->t1);
-    (void)hipFree(p->t2);
+    (void)hipFree(p->G);
+    if (p->fft) (void)hipfftDestroy((hipfftHandle)p->fft);
     (void)hipFree(p->acc);
     (void)hipFree(p->a1);
     (void)hipFree(p->a2);
@@ -799,28 +762,36 @@ extern "C" int mtip_plan_set_model(mtip_plan *p, const float *ac, void *stream)
     DeviceGuard dg(p->device);
     MTIP_REQUIRE(dg.ok, MTIP_ERR_HIP, "mtip_plan_set_model: cannot select device");
     hipStream_t st = (hipStream_t)stream;
-    const int K = p->K, np = p->np, nch = (np + kOch - 1) / kOch;
-    const size_t shm = p->n * sizeof(double2);
-    // pass 0: axis 2 (k3 -> o3); A = K*K, C = 1
-    int64_t A = (int64_t)K * K, C = 1;
-    int64_t nt = A * nch * C;
-    hipLaunchKernelGGL(k_dft_pass<0>, dim3((unsigned)mtip::ceil_div(nt, 256)), dim3(256), shm, st,
-                       ac, nullptr, p->t1, nullptr, p->dinv, p->tw, p->M, p->h, p->n, np, p->off,
-                       A, C);
-    // pass 1: axis 1 (k2 -> o2); A = K, C = np
-    A = K;
-    C = np;
-    nt = A * nch * C;
-    hipLaunchKernelGGL(k_dft_pass<1>, dim3((unsigned)mtip::ceil_div(nt, 256)), dim3(256), shm, st,
-                       nullptr, p->t1, p->t2, nullptr, p->dinv, p->tw, p->M, p->h, p->n, np,
-                       p->off, A, C);
-    // pass 2: axis 0 (k1 -> o1); A = 1, C = np * np ; real output
-    A = 1;
-    C = (int64_t)np * np;
-    nt = A * nch * C;
-    hipLaunchKernelGGL(k_dft_pass<2>, dim3((unsigned)mtip::ceil_div(nt, 256)), dim3(256), shm, st,
-                       nullptr, p->t2, nullptr, p->grid, p->dinv, p->tw, p->M, p->h, p->n, np,
-                       p->off, A, C);
+    const int64_t n3 = (int64_t)p->n * p->n * p->n;
+    const int64_t np3 = (int64_t)p->np * p->np * p->np;
+    if (!p->G) {
+        // first model on this plan: the FFT buffer and plan (plans used only
+        // for the adjoint never need them)
+        hipError_t e = hipMalloc(&p->G, (size_t)n3 * sizeof(double2));
+        if (e != hipSuccess) {
+            p->G = nullptr;
+            return mtip::fail(e == hipErrorOutOfMemory ? MTIP_ERR_NOMEM : MTIP_ERR_HIP,
+                              std::string("mtip_plan_set_model: ") + hipGetErrorString(e));
+        }
+        hipfftHandle fh = nullptr;
+        if (hipfftPlan3d(&fh, p->n, p->n, p->n, HIPFFT_Z2Z) != HIPFFT_SUCCESS) {
+            (void)hipFree(p->G);
+            p->G = nullptr;
+            return mtip::fail(MTIP_ERR_HIP, "mtip_plan_set_model: hipfftPlan3d failed");
+        }
+        p->fft = fh;
+    }
+    hipLaunchKernelGGL(k_scatter_model, dim3((unsigned)mtip::ceil_div(n3, 256)), dim3(256), 0, st,
+                       ac, p->dinv, p->G, p->M, p->h, p->n);
+    MTIP_HIP_CHECK(hipGetLastError());
+    hipfftHandle fh = (hipfftHandle)p->fft;
+    MTIP_REQUIRE(hipfftSetStream(fh, st) == HIPFFT_SUCCESS, MTIP_ERR_HIP,
+                 "mtip_plan_set_model: hipfftSetStream failed");
+    MTIP_REQUIRE(hipfftExecZ2Z(fh, (hipfftDoubleComplex *)p->G, (hipfftDoubleComplex *)p->G,
+                               HIPFFT_FORWARD) == HIPFFT_SUCCESS,
+                 MTIP_ERR_HIP, "mtip_plan_set_model: hipfftExecZ2Z failed");
+    hipLaunchKernelGGL(k_pad_grid, dim3((unsigned)mtip::ceil_div(np3, 256)), dim3(256), 0, st,
+                       p->G, p->grid, p->n, p->np, p->off);
     MTIP_HIP_CHECK(hipGetLastError());
     p->loaded = true;
     return MTIP_OK;

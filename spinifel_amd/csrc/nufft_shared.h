@@ -16,8 +16,8 @@ struct mtip_plan {
     float *grid = nullptr;        // (np)^3 real fine grid
     double *dinv = nullptr;       // K: 1 / phihat(k)
     double2 *tw = nullptr;        // n: exp(-2 pi i m / n)
-    double2 *t1 = nullptr;        // K * K * np
-    double2 *t2 = nullptr;        // K * np * np
+    double2 *G = nullptr;         // n^3: the deconvolved model at k mod n, then its FFT
+    void *fft = nullptr;          // hipfftHandle of the n^3 Z2Z transform
     bool loaded = false;
     // type-1 (adjoint) state, allocated on first use
     double *acc = nullptr;        // (np)^3 FP64 spreading grid
