@@ -218,6 +218,40 @@ int mtip_phase_amplitudes(void *spectrum, const double *amplitudes, int64_t n, v
 int mtip_phase_update(double *rho, const double *rho_mod, const uint8_t *support, int64_t n,
                       int mode, double beta, double rho_max, void *stream);
 
+/* ---------------------------------------------------------------- merge */
+
+/*
+ * Vector work of the merge's CG iterations (FP64, device pointers), around
+ * the caller's real-to-complex FFTs of
+ *   W u = s . crop(irfft(rfft(pad(s . u)) . F)) + rlambda u + flambda s . irfft(rfft(s . u) . mask)
+ * (spinifel/sequential/autocorrelation.py:110-145, :173-178).
+ * mtip_merge_pad: ups[:M, :M, :M] = s . u inside the caller-zeroed
+ * (M_ups)^3 array (the rest is left alone) and us = s . u (M^3);
+ * support may be NULL for 1.
+ * mtip_merge_combine: out = s . y_ups[:M, :M, :M] + rlambda u + flambda s . y_reg.
+ */
+int mtip_merge_pad(const double *u, const double *support, int M, int M_ups, double *ups,
+                   double *us, void *stream);
+int mtip_merge_combine(const double *y_ups, const double *y_reg, const double *u,
+                       const double *support, int M, int M_ups, double rlambda, double flambda,
+                       double *out, void *stream);
+
+/*
+ * Conjugate-gradient steps of cupyx/scipy cg (:211-212) with the step
+ * lengths on the device: mtip_dot writes a . b to *out (two fixed-order
+ * reduction passes, workspace of mtip_dot_workspace_size() bytes);
+ * mtip_cg_update: x += (rho / pq) p, r -= (rho / pq) q;
+ * mtip_cg_direction: p = r (first) or r + (rho / rho_old) p.
+ * rho, rho_old, pq: device FP64 scalars.
+ */
+size_t mtip_dot_workspace_size(void);
+int mtip_dot(const double *a, const double *b, int64_t n, double *work, size_t work_bytes,
+             double *out, void *stream);
+int mtip_cg_update(double *x, double *r, const double *p, const double *q, const double *rho,
+                   const double *pq, int64_t n, void *stream);
+int mtip_cg_direction(double *p, const double *r, const double *rho, const double *rho_old,
+                      int64_t n, int first, void *stream);
+
 /* ------------------------------------------------------------- data prep */
 
 /*

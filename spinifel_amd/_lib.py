@@ -50,6 +50,12 @@ _SIGS = {
     "mtip_phase_amplitudes": (_i32, [_vp, _vp, _i64, _vp]),
     "mtip_phase_update": (_i32, [_vp, _vp, _vp, _i64, _i32, _f64, _f64, _vp]),
     "mtip_clip_bin": (_i32, [_vp, _i32, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp]),
+    "mtip_merge_pad": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _vp]),
+    "mtip_merge_combine": (_i32, [_vp, _vp, _vp, _vp, _i32, _i32, _f64, _f64, _vp, _vp]),
+    "mtip_dot_workspace_size": (_c.c_size_t, []),
+    "mtip_dot": (_i32, [_vp, _vp, _i64, _vp, _c.c_size_t, _vp, _vp]),
+    "mtip_cg_update": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "mtip_cg_direction": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _vp]),
     "mtip_row_stats": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "mtip_scaling_ratio": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     "mtip_segment_scaling_ratio": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),

@@ -35,10 +35,12 @@ Differences a caller can observe:
   * ``solve_ac`` returns the real float64 (M, M, M) array like the
     reference; ``last`` holds the device result and the iteration count.
 """
+import ctypes
+
 import numpy as np
 import torch
 
-from . import engine, filters
+from . import _lib, engine, filters
 from .autocorrelation import get_plan, gen_nonuniform_positions
 from .quaternions import random_quaternions
 
@@ -96,6 +98,59 @@ def cg(matvec, b, x0, maxiter, tol=1e-5, callback=None):
     return x, info, it
 
 
+def cg_fused(matvec, b, x0, maxiter, tol=1e-5, callback=None):
+    """The same iteration as ``cg`` with libmtip's vector kernels: the step
+    lengths stay on the device (rho = r.r of the previous update serves as the
+    next iteration's rho), dot products reduce in a fixed order, and the only
+    host sync per iteration is the stopping test's ||r|| (cupyx's cg syncs
+    there too).  Returns (x, info, iterations)."""
+    lib = _lib.load()
+    dev = b.device
+    st = engine._stream()
+    P = engine._ptr
+    n = b.numel()
+    work = torch.empty(int(lib.mtip_dot_workspace_size()) // 8, dtype=torch.float64, device=dev)
+    wbytes = work.numel() * 8
+    sc = torch.zeros(4, dtype=torch.float64, device=dev)    # rho[0], rho[1], pq, b.b
+
+    def dot(a, c, slot):
+        _lib.check(lib.mtip_dot(P(a), P(c), n, P(work), wbytes,
+                                ctypes.c_void_p(sc.data_ptr() + 8 * slot), st), "mtip_dot")
+
+    def ptr(slot):
+        return ctypes.c_void_p(sc.data_ptr() + 8 * slot)
+
+    x = x0.clone().contiguous()
+    b = b.contiguous()
+    dot(b, b, 3)
+    b_norm = float(torch.sqrt(sc[3]))
+    if b_norm == 0.0:
+        return b.clone(), 0, 0
+    atol = tol * b_norm
+    r = (b - matvec(x)).contiguous()
+    p = torch.empty_like(r)
+    it, cur = 0, 0
+    dot(r, r, cur)
+    resid = float(torch.sqrt(sc[cur]))
+    while it < maxiter:
+        _lib.check(lib.mtip_cg_direction(P(p), P(r), ptr(cur), ptr(1 - cur), n, int(it == 0), st),
+                   "mtip_cg_direction")
+        q = matvec(p).contiguous()
+        dot(p, q, 2)
+        _lib.check(lib.mtip_cg_update(P(x), P(r), P(p), P(q), ptr(cur), ptr(2), n, st),
+                   "mtip_cg_update")
+        it += 1
+        if callback is not None:
+            callback(x)
+        cur = 1 - cur
+        dot(r, r, cur)
+        resid = float(torch.sqrt(sc[cur]))
+        if resid <= atol:
+            break
+    info = it if (it == maxiter and not resid <= atol) else 0
+    return x, info, it
+
+
 class Merge:
     """spinifel/sequential/autocorrelation.py:35-225 on the GPU."""
 
@@ -142,6 +197,7 @@ class Merge:
         self._tiling_ups = engine.auto_spread_tiling(self._plan_ups, self.det_shape)
         # zero-padded operand of the convolution; only [:M]^3 is ever written
         self._ups = torch.zeros((self.M_ups,) * 3, dtype=torch.float64, device=self.device)
+        self._us = torch.empty((self.M,) * 3, dtype=torch.float64, device=self.device)
 
         def callback(xk):
             callback.counter += 1
@@ -200,10 +256,30 @@ class Merge:
             order=self._tiling_ups)
         F_half = torch.fft.rfftn(torch.fft.ifftshift(ugrid_conv)) / self.M ** 3
 
+        lib = _lib.load()
+        P = engine._ptr
+        sup = ac_support.contiguous()
+
         def W_matvec(uvect):
-            uvect_ADA = self.core_problem_convolution(uvect, F_half, ac_support)
-            uvect_FDF = self.fourier_reg(uvect, ac_support)
-            return uvect_ADA + self.rlambda * uvect + self.flambda * uvect_FDF
+            # the same operator as core_problem_convolution + rlambda u +
+            # flambda fourier_reg, with the padding and the recombination in
+            # two kernels around the four FFTs
+            M, Mu = self.M, self.M_ups
+            u = uvect.contiguous()
+            st = engine._stream()
+            _lib.check(lib.mtip_merge_pad(P(u), P(sup), M, Mu, P(self._ups), P(self._us), st),
+                       "mtip_merge_pad")
+            X = torch.fft.rfftn(self._ups)
+            X *= F_half
+            y_ups = torch.fft.irfftn(X, s=(Mu,) * 3).contiguous()
+            R = torch.fft.rfftn(self._us)
+            R *= self._reg_mask
+            y_reg = torch.fft.irfftn(R, s=(M,) * 3).contiguous()
+            out = torch.empty(M ** 3, dtype=torch.float64, device=self.device)
+            _lib.check(lib.mtip_merge_combine(P(y_ups), P(y_reg), P(u), P(sup), M, Mu,
+                                              float(self.rlambda), float(self.flambda), P(out),
+                                              st), "mtip_merge_combine")
+            return out
 
         uvect_ADb = self._plan.adjoint_slices(
             quats, self._q, self.mult, values=self._data, support=ac_support,
@@ -232,7 +308,7 @@ class Merge:
         x0 = ac_est.reshape(-1)
         W, d = self.setup_linops(orientations, ac_support, x0)
         before = self.callback.counter
-        ret, info, it = cg(W, d, x0, self.maxiter, callback=self.callback)
+        ret, info, it = cg_fused(W, d, x0, self.maxiter, callback=self.callback)
         ac = ret.view(M, M, M)
         self.last = dict(ac=ac, info=info, iterations=self.callback.counter - before,
                          support=ac_support, d=d, W=W)

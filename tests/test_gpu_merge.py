@@ -91,11 +91,12 @@ def test_gaussian_filter_and_cg_vs_oracle(torch_dev):
     b, x0 = rng.standard_normal(50), rng.standard_normal(50)
     At = torch.as_tensor(Am, device=torch_dev)
     for maxiter in (4, 200):
-        x, info, it = merge.cg(lambda v: At @ v, torch.as_tensor(b, device=torch_dev),
-                               torch.as_tensor(x0, device=torch_dev), maxiter)
         xr, info_r, it_r = omerge.cg(lambda v: Am @ v, b, x0, maxiter)
-        assert (info, it) == (info_r, it_r)
-        np.testing.assert_allclose(x.cpu().numpy(), xr, rtol=1e-10, atol=1e-12)
+        for solver in (merge.cg, merge.cg_fused):     # torch ops, and libmtip's kernels
+            x, info, it = solver(lambda v: At @ v, torch.as_tensor(b, device=torch_dev),
+                                 torch.as_tensor(x0, device=torch_dev), maxiter)
+            assert (info, it) == (info_r, it_r)
+            np.testing.assert_allclose(x.cpu().numpy(), xr, rtol=1e-10, atol=1e-12)
 
 
 @pytest.mark.parametrize("with_estimate", [False, True])
