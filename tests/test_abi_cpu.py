@@ -113,3 +113,18 @@ def test_prep_loaders_name_h5py():
     from spinifel_amd import prep
     with pytest.raises(ImportError, match="h5py"):
         prep.load_slices(np.zeros((1, 1, 4, 4)), 0, 1, "/nonexistent.h5")
+
+
+def test_argument_errors_of_merge_entry_points():
+    lib = _lib.load()
+    nul = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.mtip_merge_pad(fake, nul, 9, 8, fake, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_merge_pad(nul, nul, 9, 18, fake, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_merge_combine(fake, fake, fake, nul, 0, 18, 0.0, 0.0, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_dot_workspace_size() >= 8
+    assert lib.mtip_dot(fake, fake, 10, fake, 8, fake, nul) == _lib.MTIP_ERR_NOMEM
+    assert lib.mtip_dot(fake, fake, -1, fake, 1 << 20, fake, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_cg_update(nul, fake, fake, fake, fake, fake, 4, nul) == _lib.MTIP_ERR_ARG
+    assert lib.mtip_cg_update(nul, nul, nul, nul, nul, nul, 0, nul) == _lib.MTIP_OK
+    assert lib.mtip_cg_direction(fake, fake, fake, nul, 4, 0, nul) == _lib.MTIP_ERR_ARG
