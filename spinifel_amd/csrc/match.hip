@@ -349,28 +349,38 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-    // staging: 2048 16-byte chunks per operand tile, 4 per thread per operand
-    uint4 ra[4], rb[4];
-    auto load_stage = [&](int64_t kb) {
+    // staging: 2048 16-byte chunks per operand tile, 4 per thread per operand,
+    // as buffer loads over the tile's own rows: rows past the matrix and
+    // columns past kbytes are out of range and read as zero, so the loads
+    // need no branches (the host keeps 256 rows x ld within 2^31 bytes)
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const int rowsA = (int)min((int64_t)BM2, NA - a0), rowsB = (int)min((int64_t)BN2, NB - b0);
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(A + a0 * lda), (short)0, (int)((int64_t)rowsA * lda), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(B + b0 * ldb), (short)0, (int)((int64_t)rowsB * ldb), 0x00020000);
+    const int colb = (tid & 7) * 16;
+    const int vA = (tid >> 3) * (int)lda + colb, vB = (tid >> 3) * (int)ldb + colb;
+    u32x4 ra[4], rb[4];
+    auto load_stage = [&](int64_t kb) __attribute__((always_inline)) {
+        const bool in_k = kb + colb < kbytes;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int c = tid + 512 * i;
-            const int r = c >> 3, col = (c & 7) * 16;
-            const int64_t gk = kb + col;
-            const int64_t ga = a0 + r, gb = b0 + r;
-            ra[i] = (ga < NA && gk < kbytes) ? load16(A + ga * lda + gk) : make_uint4(0, 0, 0, 0);
-            rb[i] = (gb < NB && gk < kbytes) ? load16(B + gb * ldb + gk) : make_uint4(0, 0, 0, 0);
+            const int oa = in_k ? vA + 64 * i * (int)lda + (int)kb : 0x7fffffff;
+            const int ob = in_k ? vB + 64 * i * (int)ldb + (int)kb : 0x7fffffff;
+            ra[i] = __builtin_amdgcn_raw_buffer_load_b128(rsA, oa, 0, 0);
+            rb[i] = __builtin_amdgcn_raw_buffer_load_b128(rsB, ob, 0, 0);
         }
     };
-    auto store_stage = [&](int buf) {
+    auto store_stage = [&](int buf) __attribute__((always_inline)) {
         uint8_t *sA = smem + buf * STAGE;
         uint8_t *sB = sA + BM2 * LDS_STRIDE;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int c = tid + 512 * i;
             const int r = c >> 3, col = (c & 7) * 16;
-            *(uint4 *)(sA + r * LDS_STRIDE + col) = ra[i];
-            *(uint4 *)(sB + r * LDS_STRIDE + col) = rb[i];
+            *(u32x4 *)(sA + r * LDS_STRIDE + col) = ra[i];
+            *(u32x4 *)(sB + r * LDS_STRIDE + col) = rb[i];
         }
     };
 
@@ -380,6 +390,13 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
     if (nstage > 1) load_stage(BKB);
     __syncthreads();
     for (int st = 0; st < nstage; ++st) {
+        // stage st + 1 (fetched during stage st - 1) goes to the other buffer,
+        // last read in stage st - 1, and stage st + 2 is fetched: both overlap
+        // this stage's MFMAs; the barrier at the end publishes stage st + 1
+        // (past the last stage these write zeros to the idle buffer and fetch
+        // nothing: the loop body stays one straight-line block)
+        store_stage((st + 1) & 1);
+        load_stage((int64_t)(st + 2) * BKB);
         const uint8_t *sA = smem + (st & 1) * STAGE;
         const uint8_t *sB = sA + BM2 * LDS_STRIDE;
         const uint8_t *ar[4], *br[2];
@@ -405,20 +422,40 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
                             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][t], bv[j][t], acc[i][j], 0, 0, 0);
             }
         } else if constexpr (PREC == MTIP_BF16) {
+            // fragments of sub-step s4 + 1 are read while sub-step s4's MFMAs run
+            bf16x8 av[2][4], bv[2][2];
+            auto frag = [&](int s4, int b) __attribute__((always_inline)) {
+                const int off = s4 * 32 + half * 16;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) av[b][i] = *(const bf16x8 *)(ar[i] + off);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[b][j] = *(const bf16x8 *)(br[j] + off);
+            };
+            frag(0, 0);
 #pragma unroll
             for (int s4 = 0; s4 < 4; ++s4) {
-                const int off = s4 * 32 + half * 16;
-                bf16x8 av[4], bv[2];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) av[i] = *(const bf16x8 *)(ar[i] + off);
-#pragma unroll
-                for (int j = 0; j < 2; ++j) bv[j] = *(const bf16x8 *)(br[j] + off);
+                if (s4 + 1 < 4) frag(s4 + 1, (s4 + 1) & 1);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[s4 & 1][i], bv[s4 & 1][j], acc[i][j], 0, 0, 0);
             }
+            // issue order: sub-step 0's fragments, then each MFMA followed by
+            // one fragment read of the next sub-step (18 in all), with the 8
+            // LDS stores and 8 loads of the staging spread over the stage
+            __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#define MTIP_SG(DSR, DSW, VM)                                                  \
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                         \
+    if (DSR) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                \
+    if (DSW) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);                \
+    if (VM) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+#define MTIP_SG4(R)  MTIP_SG(R, 1, 0) MTIP_SG(R, 0, 1) MTIP_SG(R, 0, 0) MTIP_SG(R, 0, 0)
+            MTIP_SG4(1) MTIP_SG4(1) MTIP_SG4(1) MTIP_SG4(1)
+            MTIP_SG(1, 1, 0) MTIP_SG(1, 0, 1) MTIP_SG(0, 0, 0) MTIP_SG(0, 0, 0)
+            MTIP_SG4(0) MTIP_SG4(0) MTIP_SG4(0)
+#undef MTIP_SG4
+#undef MTIP_SG
         } else {
 #pragma unroll
             for (int s2 = 0; s2 < 2; ++s2) {
@@ -434,10 +471,6 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv[j], acc[i][j], 0, 0, 0, 127, 0, 127);
                 }
             }
-        }
-        if (st + 1 < nstage) {
-            store_stage((st + 1) & 1);
-            if (st + 2 < nstage) load_stage((int64_t)(st + 2) * BKB);
         }
         __syncthreads();
     }
@@ -1025,6 +1058,12 @@ extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_
     MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM,
                  "mtip_match_partial: workspace too small");
     const int eb = prec_bytes(precision);
+    // the 256-row kernel addresses a tile's rows with 32-bit buffer offsets
+    MTIP_REQUIRE(tile_rows(precision, N_o, N_i) != BM2 ||
+                     (int64_t)BM2 * (ld_s > ld_d ? ld_s : ld_d) * eb < ((int64_t)1 << 31),
+                 MTIP_ERR_UNSUPPORTED,
+                 "mtip_match_partial: rows longer than 8 MiB need the 128-row tile "
+                 "(mtip_set_gemm_tile(precision, 128))");
     hipStream_t st = (hipStream_t)stream;
     uint64_t *part = (uint64_t *)workspace;
     const int64_t kbytes = N_p * eb;
