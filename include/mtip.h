@@ -340,9 +340,11 @@ size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision)
 
 /*
  * host: orientation rows per GEMM tile for `precision`: 0 restores the
- * built-in rule (256 for BF16 with N_o, N_i >= 1024, else 128), 128 or 256
- * force a tile.  Process-wide; set it before mtip_match_partial and keep it
- * until the matching mtip_topk_merge.  No counterpart in the reference (the
+ * built-in rule (256 for BF16 and FP8 with N_o, N_i >= 1024, else 128), 128
+ * or 256 force a tile.  Process-wide; set it before mtip_match_partial and
+ * keep it until the matching mtip_topk_merge.  The 256-row tile needs
+ * 256 * ld * element bytes < 2^31 (mtip_match_partial reports
+ * MTIP_ERR_UNSUPPORTED otherwise).  No counterpart in the reference (the
  * reference has one GEMM path); the project's autotuner sets it from timings.
  */
 int mtip_set_gemm_tile(int precision, int rows);

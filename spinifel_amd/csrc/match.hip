@@ -471,6 +471,20 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv[j], acc[i][j], 0, 0, 0, 127, 0, 127);
                 }
             }
+            // issue order: sub-step 0's fragment reads, then each MFMA followed
+            // by reads of sub-step 1 (12 over the first 8 MFMAs), with the
+            // staging's 8 LDS stores and then its 8 loads spread over the stage
+            __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);
+#define MTIP_SG(DSR, DSW, VM)                                                  \
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                         \
+    if (DSR) __builtin_amdgcn_sched_group_barrier(0x100, DSR, 0);              \
+    if (DSW) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);                \
+    if (VM) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            MTIP_SG(2, 1, 0) MTIP_SG(1, 1, 0) MTIP_SG(2, 1, 0) MTIP_SG(1, 1, 0)
+            MTIP_SG(2, 1, 0) MTIP_SG(1, 1, 0) MTIP_SG(2, 1, 0) MTIP_SG(1, 1, 0)
+            MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1)
+            MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1) MTIP_SG(0, 0, 1)
+#undef MTIP_SG
         }
         __syncthreads();
     }
@@ -592,8 +606,14 @@ __global__ __launch_bounds__(256) void k_rescore(const float *__restrict__ S, in
     __shared__ double s64[4][K];
     __shared__ float s32[4][K];
     // images in the caller's order (e.g. grouped by best candidate, so that
-    // neighbouring workgroups read the same candidate slices)
-    const int64_t i = order ? (int64_t)order[blockIdx.x] : (int64_t)blockIdx.x;
+    // neighbouring workgroups read the same candidate slices); workgroups go
+    // round-robin over the 8 XCDs, so each XCD takes a contiguous run of the
+    // order and the images resident on it share its L2
+    int64_t i = blockIdx.x;
+    if (order) {
+        const int64_t nb = gridDim.x, n8 = nb / 8, b = blockIdx.x;
+        i = (int64_t)order[b < 8 * n8 ? (b % 8) * n8 + b / 8 : b];
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double al = alpha_per_image ? alpha[i] : alpha[0];
     const float alf = (float)al;
@@ -944,15 +964,15 @@ __global__ __launch_bounds__(256) void k_col_finish(const double *__restrict__ w
 std::atomic<int> g_tile_override[3] = {{0}, {0}, {0}};
 
 // Orientation rows per top-K partial tile: 256 (big kernel) for large BF16
-// problems, else 128, unless overridden.  The partial buffer and the merge
-// use the same value.
+// and FP8 problems, else 128, unless overridden.  The partial buffer and the
+// merge use the same value.
 int tile_rows(int precision, int64_t N_o, int64_t N_i)
 {
     if (precision >= 0 && precision < 3) {
         const int o = g_tile_override[precision].load(std::memory_order_relaxed);
         if (o) return o;
     }
-    return (precision == MTIP_BF16 && N_o >= 1024 && N_i >= 1024) ? BM2 : BM;
+    return (precision != MTIP_FP32 && N_o >= 1024 && N_i >= 1024) ? BM2 : BM;
 }
 
 template <int PREC, int K>

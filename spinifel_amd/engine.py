@@ -152,6 +152,22 @@ def _orient_order(quats):
     return perm
 
 
+def _orient_rank(quats):
+    """Position of each orientation in ``_orient_order(quats)`` (int32),
+    cached on the tensor like the order."""
+    hit = getattr(quats, "_mtip_orient_rank", None)
+    if hit is not None and hit[0] == quats._version:
+        return hit[1]
+    perm = _orient_order(quats).long()
+    rank = torch.empty(perm.numel(), dtype=torch.int32, device=perm.device)
+    rank[perm] = torch.arange(perm.numel(), dtype=torch.int32, device=perm.device)
+    try:
+        quats._mtip_orient_rank = (quats._version, rank)
+    except AttributeError:
+        pass
+    return rank
+
+
 def auto_tiling(plan, det_shape):
     """Tiling by rule, without timing: 16 x 16 blocks while a block's fine-grid
     box fits LDS (fine grids up to ~200 points per edge at 0.9 Nyquist), 8 x 8
@@ -616,6 +632,14 @@ class SliceSet:
             self._center = col_mean(self.f32, self.N_p)
         return self._center
 
+    @property
+    def rank(self):
+        """Position of each orientation in the Morton order of the slicing
+        (int32, N_o), or None when the orientations are not known."""
+        if self.quats is None or self.N_o < 64:
+            return None
+        return _orient_rank(self.quats)
+
     def operand(self, precision, center=True, stream=None):
         """Slices minus the centre (or plain), cast to `precision`, cached."""
         key = (precision, bool(center))
@@ -663,7 +687,16 @@ def match_resident(slices, images, precision="fp32", K=None, rescore_fp64=None, 
                     K, workspace, stream, timers)
     out = dict(topk_idx=idx, topk_val=val, alpha=alpha)
     if rescore_fp64:
-        r = rescore(slices.f32, images.f32, images.N_p, alpha, idx, stream, timers)
+        order = None
+        if slices.rank is not None:
+            # images whose best candidates are neighbours in orientation space
+            # go to neighbouring workgroups, so their candidate slices are
+            # re-read from cache rather than from HBM
+            with _span(timers, "order"):
+                key = slices.rank[idx[:, 0].long().clamp(0, slices.N_o - 1)]
+                order = torch.argsort(key).to(torch.int32)
+        r = rescore(slices.f32, images.f32, images.N_p, alpha, idx, stream, timers,
+                    image_order=order)
         out.update(index=r["best"], best_d64=r["best_d64"], d32=r["d32"], d64=r["d64"])
     else:
         out["index"] = idx[:, 0]
