@@ -806,106 +806,108 @@ __global__ __launch_bounds__(256) void k_residual(const float *__restrict__ in, 
         const double mv = (invert & 2) ? mult[r] : mult[0];
         m = (invert & 1) ? (float)(1.0 / mv) : (float)mv;
     }
+    // Thread t owns columns 4t + 1024 k + (0..3).  Each trip handles four such
+    // groups, k = 4j .. 4j + 3, with all their loads issued before any use, so
+    // a block keeps 16 KB of its row in flight; sums keep the column order.
+    constexpr int kGroups = 4;
+    const int64_t cols4 = (cols / 4) * 4;
+    // x = m src - centre over columns c4 .. c4 + 3 (zero at and past cols)
+    auto load4 = [&](int64_t c4, float (&x)[4]) __attribute__((always_inline)) {
+        if (vec && c4 + 4 <= cols4) {
+            const float4 v = *reinterpret_cast<const float4 *>(src + c4);
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            if (center) {
+                const float4 cc = *reinterpret_cast<const float4 *>(center + c4);
+                const float cv[4] = {cc.x, cc.y, cc.z, cc.w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[t] = fmaf(m, vv[t], -cv[t]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[t] = m * vv[t];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int64_t c = c4 + t;
+                x[t] = c < cols ? (center ? fmaf(m, src[c], -center[c]) : m * src[c]) : 0.0f;
+            }
+        }
+    };
+    const int64_t c_first = 4 * (int64_t)threadIdx.x;
     float scale = 1.0f, inv = 1.0f;
     if (FMT == MTIP_FP8) {
         float a = 0.0f;
-        for (int64_t c = threadIdx.x; c < cols; c += 256)
-            a = fmaxf(a, fabsf(center ? fmaf(m, src[c], -center[c]) : m * src[c]));
+        for (int64_t c0 = c_first; c0 < cols; c0 += 1024 * kGroups) {
+            float x[kGroups][4];
+#pragma unroll
+            for (int u = 0; u < kGroups; ++u) load4(c0 + 1024 * u, x[u]);
+#pragma unroll
+            for (int u = 0; u < kGroups; ++u)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) a = fmaxf(a, fabsf(x[u][t]));
+        }
         a = block_max_f32(a, redf);
         scale = (a > 0.0f && isfinite(a)) ? a / 448.0f : 1.0f;
         inv = 1.0f / scale;
         if (threadIdx.x == 0 && scales) scales[r] = scale;
     }
     double ss = 0.0;
-    if (FMT == MTIP_FP8) {
-        uint8_t *dst = (uint8_t *)out + r * ld_out;
-        for (int64_t c4 = 4 * (int64_t)threadIdx.x; c4 < ld_out; c4 += 1024) {
-            float v[4];
+    for (int64_t c0 = c_first; c0 < ld_out; c0 += 1024 * kGroups) {
+        float x[kGroups][4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                int64_t c = c4 + t;
-                float x = 0.0f;
-                if (c < cols) x = (center ? fmaf(m, src[c], -center[c]) : m * src[c]) * inv;
-                v[t] = fminf(fmaxf(x, -448.0f), 448.0f);
-            }
-            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
-            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
-            const float dqv[4] = {__builtin_amdgcn_cvt_f32_fp8(pk, 0), __builtin_amdgcn_cvt_f32_fp8(pk, 1),
-                                  __builtin_amdgcn_cvt_f32_fp8(pk, 2), __builtin_amdgcn_cvt_f32_fp8(pk, 3)};
+        for (int u = 0; u < kGroups; ++u) load4(c0 + 1024 * u, x[u]);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (c4 + t < ld_out) {
-                    dst[c4 + t] = (uint8_t)((uint32_t)pk >> (8 * t));
-                    double dq = (double)dqv[t] * (double)scale;
-                    ss += dq * dq;
-                }
-            }
-        }
-    } else if (!vec) {
-        for (int64_t c = threadIdx.x; c < ld_out; c += 256) {
-            float x = 0.0f;
-            if (c < cols) x = center ? fmaf(m, src[c], -center[c]) : m * src[c];
-            if (FMT == MTIP_BF16) {
-                uint16_t b = bf16_rne(x);
-                ((uint16_t *)out)[r * ld_out + c] = b;
-                double dq = bf16_to_f32(b);
-                ss += dq * dq;
-            } else {
-                ((float *)out)[r * ld_out + c] = x;
-                ss += (double)x * (double)x;
-            }
-        }
-    } else {
-        // four columns per thread and instruction: 16-byte loads of the row and
-        // the centre, 8- (bf16) or 16-byte (fp32) stores, for 16-byte-aligned
-        // rows (the caller checks); the ragged tail goes element by element
-        const int64_t cols4 = (cols / 4) * 4;
-        for (int64_t c4 = 4 * (int64_t)threadIdx.x; c4 < ld_out; c4 += 1024) {
-            float x[4];
-            if (c4 + 4 <= cols4) {
-                const float4 v = *reinterpret_cast<const float4 *>(src + c4);
-                const float vv[4] = {v.x, v.y, v.z, v.w};
-                if (center) {
-                    const float4 cc = *reinterpret_cast<const float4 *>(center + c4);
-                    const float cv[4] = {cc.x, cc.y, cc.z, cc.w};
+        for (int u = 0; u < kGroups; ++u) {
+            const int64_t c4 = c0 + 1024 * u;
+            if (c4 >= ld_out) break;
+            const bool full = c4 + 4 <= ld_out;
+            if (FMT == MTIP_FP8) {
+                float v[4];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) x[t] = fmaf(m, vv[t], -cv[t]);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) x[t] = m * vv[t];
-                }
-            } else {
+                for (int t = 0; t < 4; ++t) v[t] = fminf(fmaxf(x[u][t] * inv, -448.0f), 448.0f);
+                int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+                pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+                const float dqv[4] = {__builtin_amdgcn_cvt_f32_fp8(pk, 0), __builtin_amdgcn_cvt_f32_fp8(pk, 1),
+                                      __builtin_amdgcn_cvt_f32_fp8(pk, 2), __builtin_amdgcn_cvt_f32_fp8(pk, 3)};
+                uint8_t *d = (uint8_t *)out + r * ld_out + c4;
+                if (vec && full) *reinterpret_cast<uint32_t *>(d) = (uint32_t)pk;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const int64_t c = c4 + t;
-                    x[t] = c < cols ? (center ? fmaf(m, src[c], -center[c]) : m * src[c]) : 0.0f;
+                    if (c4 + t < ld_out) {
+                        if (!(vec && full)) d[t] = (uint8_t)((uint32_t)pk >> (8 * t));
+                        const double dq = (double)dqv[t] * (double)scale;
+                        ss += dq * dq;
+                    }
                 }
-            }
-            if (FMT == MTIP_BF16) {
+            } else if (FMT == MTIP_BF16) {
                 uint16_t b[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    b[t] = bf16_rne(x[t]);
-                    const double dq = bf16_to_f32(b[t]);
-                    ss += dq * dq;
-                }
+                for (int t = 0; t < 4; ++t) b[t] = bf16_rne(x[u][t]);
                 uint16_t *d = (uint16_t *)out + r * ld_out + c4;
-                if (c4 + 4 <= ld_out) {
+                if (vec && full) {
                     uint2 pk;
                     pk.x = (uint32_t)b[0] | ((uint32_t)b[1] << 16);
                     pk.y = (uint32_t)b[2] | ((uint32_t)b[3] << 16);
                     *reinterpret_cast<uint2 *>(d) = pk;
-                } else {
-                    for (int t = 0; t < 4 && c4 + t < ld_out; ++t) d[t] = b[t];
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (c4 + t < ld_out) {
+                        if (!(vec && full)) d[t] = b[t];
+                        const double dq = bf16_to_f32(b[t]);
+                        ss += dq * dq;
+                    }
                 }
             } else {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) ss += (double)x[t] * (double)x[t];
                 float *d = (float *)out + r * ld_out + c4;
-                if (c4 + 4 <= ld_out) {
-                    *reinterpret_cast<float4 *>(d) = make_float4(x[0], x[1], x[2], x[3]);
-                } else {
-                    for (int t = 0; t < 4 && c4 + t < ld_out; ++t) d[t] = x[t];
+                if (vec && full)
+                    *reinterpret_cast<float4 *>(d) = make_float4(x[u][0], x[u][1], x[u][2], x[u][3]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (c4 + t < ld_out) {
+                        if (!(vec && full)) d[t] = x[u][t];
+                        ss += (double)x[u][t] * (double)x[u][t];
+                    }
                 }
             }
         }
