@@ -53,6 +53,22 @@ def test_tile_order_is_a_permutation(shape):
     assert engine.tile_order((100,)) is None
 
 
+def test_tile_order_groups_neighbours():
+    # inside a full 16 x 16 tile, every run of 4 slots is a 2 x 2 pixel block
+    shape = (4, 32, 128)
+    o = engine.tile_order(shape)
+    for run in o.reshape(-1, 4):
+        p, r, c = np.unravel_index(run, shape)
+        assert len(set(p)) == 1
+        assert r.min() % 2 == 0 and c.min() % 2 == 0
+        np.testing.assert_array_equal(r - r.min(), [0, 0, 1, 1])
+        np.testing.assert_array_equal(c - c.min(), [0, 1, 0, 1])
+    # group=None keeps row-major tiles
+    o_row = engine.tile_order(shape, group=None)
+    np.testing.assert_array_equal(o_row[:16], np.arange(16))
+    np.testing.assert_array_equal(np.sort(o_row), np.sort(o))
+
+
 def test_settings_shim():
     s = st.SpinifelSettings()
     assert s.Mquat == 20 and s.M == 81
