@@ -498,12 +498,23 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
         const int bx = any ? hi[0] - lo[0] + W : 0, by = any ? hi[1] - lo[1] + W : 0;
         const int bz = any ? hi[2] - lz + 2 * GP : 0;
         const int S = pair_stride(bz);
-        const bool staged = any && S > 0 && bx * by * S <= kBoxCap;
+        // plane stride: by * S raised to 10 mod 32 floats when the box still
+        // fits.  A quarter-wave serves a 2 x 2 pixel group whose window
+        // corners differ by about one cell per axis; with 32 banks, neighbours
+        // one plane apart then read other banks.  Measured over residues at
+        // M = 81, 10k x 16k: 8.75 ms at 10 mod 32 against 8.98 ms unpadded (a
+        // quarter-wave bank model ranks the residues alike; 0 and 6 are worst)
+        int plane = by * S;
+        {
+            const int padded = plane + ((10 - plane) & 31);
+            if (bx * padded <= kBoxCap) plane = padded;
+        }
+        const bool staged = any && S > 0 && bx * plane <= kBoxCap;
         // this point's pair base (box-local when staged, padded grid otherwise)
         int lb = -1;
         if (ok) {
             const int zb = ijk[2] & ~1;
-            lb = staged ? ((ijk[0] - lo[0]) * by + (ijk[1] - lo[1])) * S + (zb - lz)
+            lb = staged ? (ijk[0] - lo[0]) * plane + (ijk[1] - lo[1]) * S + (zb - lz)
                         : (ijk[0] * np + ijk[1]) * np + zb;
             lb |= ijk[2] & 1;
         }
@@ -543,7 +554,8 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
                     const int k = (u0 + u) * 256 + tid;
                     if (k < nchunk) {
                         const int r = (int)(((float)k + 0.5f) * inv_q4), z4 = 4 * (k - r * q4);
-                        float2 *d2 = reinterpret_cast<float2 *>(s_box + r * S + z4);
+                        const int xr = (int)(((float)r + 0.5f) * inv_by), yr = r - xr * by;
+                        float2 *d2 = reinterpret_cast<float2 *>(s_box + xr * plane + yr * S + z4);
                         d2[0] = make_float2(__uint_as_float(v[u][0]), __uint_as_float(v[u][1]));
                         if (z4 + 2 < S)
                             d2[1] = make_float2(__uint_as_float(v[u][2]), __uint_as_float(v[u][3]));
@@ -556,7 +568,6 @@ __global__ __launch_bounds__(256, 3) void k_slices_lds(GridView gv, const double
         }
         __syncthreads();
         // ---- taps: each wave its PW points, PPW per step
-        const int plane = by * S;
 #pragma unroll 1
         for (int it = 0; it < STEPS; ++it) {
             const bool slot_ok = PW >= PPW || g < PW;
