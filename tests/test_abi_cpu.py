@@ -61,6 +61,20 @@ def test_argument_errors_are_reported_without_a_gpu():
         _lib.check(_lib.MTIP_ERR_NOMEM, "x")
 
 
+def test_long_rows_need_the_128_row_tile():
+    # the 256-row GEMM tile addresses its rows with 32-bit offsets: 256 rows of
+    # 2^22 bf16 values are 2^31 bytes, reported before any GPU work
+    lib = _lib.load()
+    fake = ctypes.c_void_p(0x1000)    # never dereferenced
+    nul = ctypes.c_void_p()
+    n, ld = 1024, 1 << 22
+    ws = lib.mtip_match_workspace_size(n, n, 8, 1)
+    assert lib.mtip_gemm_tile(1, n, n) == 256
+    rc = lib.mtip_match_partial(1, fake, ld, nul, fake, n, fake, ld, nul, n, 16, fake, 8, fake, ws,
+                                nul)
+    assert rc == _lib.MTIP_ERR_UNSUPPORTED and "128-row tile" in _lib.last_error()
+
+
 def test_product_path_fails_loudly_without_gpu():
     import torch
     if torch.cuda.is_available():
