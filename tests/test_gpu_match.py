@@ -97,6 +97,39 @@ def test_topk_sets_and_rescore(dev, precision, shape):
         np.testing.assert_allclose(v, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
+def test_topk_256_row_tile(dev, precision):
+    # the 256 x 256 kernel on ragged sizes: partial row tiles on both operands
+    # and a pixel count (a multiple of 16, as the ABI asks) that leaves a
+    # partial 128-byte K step in every precision
+    import torch
+    from spinifel_amd import _lib, engine
+    N_o, N_i, N_p = 700, 300, 1008
+    print(f"case: topk 256-row tile {precision} {(N_o, N_i, N_p)}", flush=True)
+    rng = np.random.default_rng(11)
+    S = rng.gamma(2.0, 1.0, (N_o, N_p))
+    Dm = rng.poisson(S[rng.integers(0, N_o, N_i)] * 1.3).astype(np.float64)
+    ss = _sliceset(S, dev)
+    im = engine.ImageSet(Dm, dev)
+    a = torch.tensor([1.3], dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    code = _lib.PRECISIONS[precision]
+    _lib.check(lib.mtip_set_gemm_tile(code, 256), "mtip_set_gemm_tile")
+    try:
+        assert lib.mtip_gemm_tile(code, N_o, N_i) == 256
+        r = engine.match_resident(ss, im, precision, K=8, rescore_fp64=True, alpha=a)
+        best = r["index"].cpu().numpy()
+    finally:
+        _lib.check(lib.mtip_set_gemm_tile(code, 0), "mtip_set_gemm_tile")
+    ref_idx, _ = matching.topk_squared(1.3 * S.astype(np.float32).astype(np.float64),
+                                       Dm.astype(np.float32).astype(np.float64), 8)
+    assert (best == ref_idx[:, 0]).all()
+    if precision == "fp32":
+        same = np.mean([len(set(c) & set(e)) / 8
+                        for c, e in zip(r["topk_idx"].cpu().numpy(), ref_idx)])
+        assert same >= 0.99
+
+
 def test_ties_lowest_index(dev):
     import torch
     from spinifel_amd import engine
