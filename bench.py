@@ -481,20 +481,30 @@ def run_merge(args, torch, engine, synth, log):
     out = dict(workload=f"M={M} (M_ups={settings.M_ups}), {n_img} images x 1x128x128 px "
                         f"= {npts / 1e6:.1f}M points, eps {args.eps}, CG maxiter "
                         f"{settings.solve_ac_maxiter}")
-    # ---- spreading kernel and transform passes (HIP events on the launch stream)
+    # ---- spatial order of the points, spreading kernel and transform passes
+    # (HIP events on the launch stream), as solve_ac runs them
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    for name, plan, vals, tiling in (("data_M81", mg._plan, mg._data, mg._tiling),
-                                     ("ones_M162", mg._plan_ups, None, mg._tiling_ups)):
-        ords = torch.as_tensor(tiling.order).to(dev)
+    t_ord = []
+    for r in range(args.warmup + args.steps):
+        ev[0].record()
+        (x, y, z), (xd, yd, zd, vd) = mg._points(quats)
+        ev[1].record()
+        torch.cuda.synchronize()
+        if r >= args.warmup:
+            t_ord.append(ev[0].elapsed_time(ev[1]))
+    out["order_ms"] = float(np.mean(t_ord))
+    log("merge order", out["order_ms"])
+    for name, plan, pts, vals in (("data_M81", mg._plan, (xd, yd, zd), vd),
+                                  ("ones_M162", mg._plan_ups, (x, y, z), None)):
         res_buf = torch.empty((plan.M,) * 3, dtype=torch.float64, device=dev)
         bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        n_sp = int(pts[0].numel())
         t_sp, t_fi = [], []
         for r in range(args.warmup + args.steps):
             _lib.check(lib.mtip_plan_spread_begin(plan._h, st))
             ev[0].record()
-            _lib.check(lib.mtip_plan_spread_slices(
-                plan._h, P(quats), n_img, P(q), N_p, P(ords), ords.numel(), tiling.points,
-                float(mg.mult), P(vals), N_p if vals is not None else 0, P(bad), st))
+            _lib.check(lib.mtip_plan_spread_points(plan._h, P(pts[0]), P(pts[1]), P(pts[2]), n_sp,
+                                                   P(vals), P(bad), st))
             ev[1].record()
             _lib.check(lib.mtip_plan_spread_finish(plan._h, None, 1.0 / plan.M ** 3, P(res_buf),
                                                    st))
@@ -505,18 +515,19 @@ def run_merge(args, torch, engine, synth, log):
                 t_fi.append(ev[1].elapsed_time(ev[2]))
         sp, fi = float(np.mean(t_sp)), float(np.mean(t_fi))
         npad = plan.padded_edge
-        # algorithmic HBM bytes of one spread launch: the values (4 B per point;
-        # none for ones) and one FP64 write of the padded grid
-        alg = (4 * npts if vals is not None else 0) + 8 * npad ** 3
-        lds = 2 * 4 * npts * plan.width ** 3        # read + write of every tap in LDS
-        out[name] = dict(spread_ms=sp, finish_ms=fi, tile=list(tiling.shape),
-                         points_per_s=npts / (sp * 1e-3),
+        # algorithmic HBM bytes of one spread launch: the points (24 B) and
+        # values (4 B) read once and one FP64 write of the padded grid
+        alg = (28 if vals is not None else 24) * n_sp + 8 * npad ** 3
+        lds = 2 * 4 * n_sp * plan.width ** 3        # read + write of every tap in LDS
+        out[name] = dict(spread_ms=sp, finish_ms=fi, points=n_sp, tile="128 points, Morton order",
+                         points_per_s=n_sp / (sp * 1e-3),
                          hbm_alg_bytes=alg, hbm_alg_gbps=alg / (sp * 1e-3) / 1e9,
                          hbm_frac=alg / (sp * 1e-3) / 8e12,
                          lds_tap_bytes=lds, lds_tap_tbps=lds / (sp * 1e-3) / 1e12,
                          lds_frac=lds / (sp * 1e-3) / (256 * 128 * 2.4e9),
                          bad=int(bad.item()))
         log("merge", name, json.dumps(out[name]))
+    del x, y, z, xd, yd, zd, vd
     # ---- whole solve_ac (positions on the fly, two type-1 transforms, CG)
     t_all, its = [], []
     for r in range(max(1, args.warmup // 2) + 2):

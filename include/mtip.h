@@ -182,9 +182,9 @@ int mtip_plan_spread_slices(mtip_plan *plan, const double *quats, int64_t N_o, c
 /*
  * Spread values (device FP32 (N), NULL = ones) at arbitrary points
  * (x, y, z device FP64 (N), radians): nufft_3d_t1(H_, K_, L_, nuvect, ...)
- * of spinifel/autocorrelation.py:119 / extern/nufft_ext.py:316.  Tiles of 256
+ * of spinifel/autocorrelation.py:119 / extern/nufft_ext.py:316.  Tiles of 128
  * consecutive points share an LDS box, so neighbouring points should be
- * adjacent (as a flattened detector is).
+ * adjacent (as a flattened detector is, or points put in a spatial order).
  */
 int mtip_plan_spread_points(mtip_plan *plan, const double *x, const double *y, const double *z,
                             int64_t N, const float *values, int *bad_count, void *stream);

@@ -20,7 +20,7 @@
 // has period n in l.
 //
 // Spreading: a workgroup takes a tile of points (a detector tile of one
-// orientation, or 256 consecutive caller points), bounds the union of their
+// orientation, or 128 consecutive caller points), bounds the union of their
 // windows, accumulates the tile in an LDS box and adds the box's non-zero
 // cells to the FP64 grid with FP64 global atomics (FP64 keeps the sum over
 // all tiles exact to double rounding; each LDS cell sums at most a few
@@ -479,13 +479,15 @@ extern "C" int mtip_plan_spread_points(mtip_plan *p, const double *x, const doub
     MTIP_REQUIRE(N >= 0, MTIP_ERR_ARG, "mtip_plan_spread_points: N < 0");
     if (N == 0) return MTIP_OK;
     MTIP_REQUIRE(x && y && z, MTIP_ERR_ARG, "mtip_plan_spread_points: null pointer");
-    MTIP_REQUIRE(mtip::ceil_div(N, 256) <= 0x7fffffff, MTIP_ERR_ARG,
+    MTIP_REQUIRE(mtip::ceil_div(N, 128) <= 0x7fffffff, MTIP_ERR_ARG,
                  "mtip_plan_spread_points: N too large");
     DeviceGuard dg(p->device);
     GridView gv = view_of(p);
     const double gscale = (double)p->n / (2.0 * M_PI);
+    // 128-point tiles: the kernel weights of a tile's points are computed once
+    // into LDS (256-point tiles recompute them in every wave)
     MTIP_W_DISPATCH(p->w, launch_spread, gv, p->acc, (int)GEOM_POINTS, nullptr, (int64_t)0,
-                    nullptr, (int64_t)0, nullptr, (int64_t)0, 256, x, y, z, N, gscale, values,
+                    nullptr, (int64_t)0, nullptr, (int64_t)0, 128, x, y, z, N, gscale, values,
                     (int64_t)0, bad_count, (hipStream_t)stream);
     MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;

@@ -401,6 +401,44 @@ class NufftPlan:
         return self._adjoint_finish(support, out_scale, out, bad, check_domain, stream)
 
 
+def rotated_points(quats, q, scale, stream=None):
+    """x, y, z (radians, FP64 device vectors of N_o * N_p, orientation-major)
+    of the pixel q-vectors (3, N_p) rotated by inv(R(quat)) and scaled."""
+    lib = _lib.load()
+    N_o, N_p = quats.shape[0], q.shape[1]
+    outs = [torch.empty(N_o * N_p, dtype=torch.float64, device=q.device) for _ in range(3)]
+    st = _stream(stream)
+    for s in range(0, N_o, 65535):
+        e = min(N_o, s + 65535)
+        _lib.check(lib.mtip_rotate_positions(
+            ctypes.c_void_p(quats[s:].data_ptr()), e - s, _ptr(q), N_p, float(scale),
+            ctypes.c_void_p(outs[0][s * N_p:].data_ptr()),
+            ctypes.c_void_p(outs[1][s * N_p:].data_ptr()),
+            ctypes.c_void_p(outs[2][s * N_p:].data_ptr()), st), "mtip_rotate_positions")
+    return outs
+
+
+def _part1by2(v):
+    # the low 10 bits of v spread to every third bit (bit interleaving)
+    v = v & 0x3FF
+    v = (v | (v << 16)) & 0x030000FF
+    v = (v | (v << 8)) & 0x0300F00F
+    v = (v | (v << 4)) & 0x030C30C3
+    v = (v | (v << 2)) & 0x09249249
+    return v
+
+
+def spatial_order(x, y, z, cell):
+    """Permutation (int64) visiting points (radians, |x| < 3 pi) in Morton
+    order of the cubes of edge `cell` radians they fall in: runs of
+    consecutive points then share small boxes of the fine grid, so the
+    spreading kernel's 128-point tiles accumulate compact boxes in LDS."""
+    idx = [torch.clamp(((c + 3 * np.pi) / cell).floor().to(torch.int32), 0, 1023)
+           for c in (x, y, z)]
+    key = _part1by2(idx[0]) | (_part1by2(idx[1]) << 1) | (_part1by2(idx[2]) << 2)
+    return torch.argsort(key)
+
+
 def row_stats(x, n_cols=None, stream=None):
     """FP64 per-row sum and sum of squares of a float32 (rows, ld) tensor."""
     lib = _lib.load()
