@@ -69,6 +69,26 @@ def test_tile_order_groups_neighbours():
     np.testing.assert_array_equal(np.sort(o_row), np.sort(o))
 
 
+def test_spatial_order_groups_cubes():
+    import torch
+    rng = np.random.default_rng(5)
+    pts = torch.as_tensor(rng.uniform(-3, 3, (3, 5000)))
+    cell = 0.5
+    perm = engine.spatial_order(pts[0], pts[1], pts[2], cell)
+    assert sorted(perm.tolist()) == list(range(5000))
+    # every cube's points form one contiguous run of the order
+    cube = ((pts + 3 * np.pi) / cell).floor().long()
+    key = (cube[0] * 1024 + cube[1]) * 1024 + cube[2]
+    k = key[perm].numpy()
+    starts = np.flatnonzero(np.r_[True, k[1:] != k[:-1]])
+    assert len(starts) == len(np.unique(k))
+    # Morton order: the first 8 cubes of a 2x2x2 block come before the next block
+    c = cube[:, perm].numpy()
+    blk = (c[0] // 2) * 1_000_000 + (c[1] // 2) * 1000 + (c[2] // 2)
+    b_starts = np.flatnonzero(np.r_[True, blk[1:] != blk[:-1]])
+    assert len(b_starts) == len(np.unique(blk))
+
+
 def test_settings_shim():
     s = st.SpinifelSettings()
     assert s.Mquat == 20 and s.M == 81
