@@ -380,9 +380,15 @@ class NufftPlan:
         return self._adjoint_finish(support, out_scale, out, bad, check_domain, stream)
 
     def adjoint_points(self, x, y, z, values=None, support=None, out_scale=None, out=None,
-                       stream=None, check_domain=True):
+                       stream=None, check_domain=True, spatial="auto"):
         """Re of the type-1 transform (isign +1) of values at points (x, y, z)
-        (radians) -> (M, M, M) float64, times support and out_scale (1 / M^3)."""
+        (radians) -> (M, M, M) float64, times support and out_scale (1 / M^3).
+
+        spatial: put the points in Morton order of 4-cell cubes of the fine
+        grid before spreading ('auto': from 2^20 points on), so that the
+        kernel's runs of 128 points fill compact LDS boxes whatever order the
+        caller's points come in (orientation-major detector rows make long,
+        thin boxes).  The result is the same sum, added in another order."""
         xs = [torch.as_tensor(v, dtype=torch.float64).reshape(-1).to(self.device).contiguous()
               for v in (x, y, z)]
         N = xs[0].numel()
@@ -392,6 +398,13 @@ class NufftPlan:
             values = torch.as_tensor(values).reshape(-1).to(self.device, torch.float32).contiguous()
             if values.numel() != N:
                 raise ValueError(f"{values.numel()} values for {N} points")
+        if spatial == "auto":
+            spatial = N >= (1 << 20)
+        if spatial:
+            perm = spatial_order(xs[0], xs[1], xs[2], 4 * 2 * np.pi / self.n_fine)
+            xs = [v[perm] for v in xs]
+            if values is not None:
+                values = values[perm]
         bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         st = _stream(stream)
         _lib.check(self.lib.mtip_plan_spread_begin(self._h, st), "mtip_plan_spread_begin")

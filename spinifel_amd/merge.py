@@ -266,7 +266,7 @@ class Merge:
         quats = self._quats(orientations)
         (x, y, z), data_pts = self._points(quats)
         ugrid_conv = self._plan_ups.adjoint_points(x, y, z, values=None,
-                                                   out_scale=1.0 / self.M_ups ** 3)
+                                                   out_scale=1.0 / self.M_ups ** 3, spatial=False)
         del x, y, z
         F_half = torch.fft.rfftn(torch.fft.ifftshift(ugrid_conv)) / self.M ** 3
 
@@ -297,7 +297,8 @@ class Merge:
 
         xd, yd, zd, vd = data_pts
         uvect_ADb = self._plan.adjoint_points(xd, yd, zd, values=vd, support=ac_support,
-                                              out_scale=1.0 / self.M ** 3).reshape(-1)
+                                              out_scale=1.0 / self.M ** 3,
+                                              spatial=False).reshape(-1)
         d = uvect_ADb + self.rlambda * x0
         return W_matvec, d
 

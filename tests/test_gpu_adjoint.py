@@ -38,17 +38,19 @@ def _oracle_adjoint(vals, H, K, L, M, support=None):
     return es_nufft.adjoint(np.asarray(vals, np.float64).ravel(), H, K, L, M, True, support)
 
 
+@pytest.mark.parametrize("spatial", [False, True])
 @pytest.mark.parametrize("M", [16, 17])
-def test_points_vs_brute_force(torch_dev, M):
-    """Scattered points (boxes too large for LDS: the direct FP64-atomic path)
-    and points beyond pi (periodic wrap)."""
+def test_points_vs_brute_force(torch_dev, M, spatial):
+    """Scattered points (in caller order the boxes are too large for LDS: the
+    direct FP64-atomic path; in spatial order, compact LDS boxes) and points
+    beyond pi (periodic wrap)."""
     from spinifel_amd import engine
-    print(f"case: adjoint_points M={M} 4000 random points", flush=True)
+    print(f"case: adjoint_points M={M} 4000 random points spatial={spatial}", flush=True)
     rng = np.random.default_rng(M)
     x = rng.uniform(-2.9 * np.pi, 2.9 * np.pi, (3, 4000))
     v = rng.standard_normal(4000).astype(np.float32)
     plan = engine.NufftPlan(M, 1e-6, 0)
-    out = plan.adjoint_points(*x, values=v).cpu().numpy()
+    out = plan.adjoint_points(*x, values=v, spatial=spatial).cpu().numpy()
     ref = es_nufft.nudft3d1(*x, v.astype(np.float64), M, 1).real / M ** 3
     assert out.shape == (M,) * 3
     assert rel_l2(out, ref) < TOL
@@ -179,3 +181,22 @@ def test_dropin_adjoint_matches_oracle(torch_dev):
     ref = _oracle_adjoint(v.astype(np.float32), H, K, L, M, sup)
     assert isinstance(out, np.ndarray) and out.dtype == np.float64 and out.shape == (M,) * 3
     assert rel_l2(out, ref) < TOL
+
+
+def test_points_spatial_order_large(torch_dev):
+    """2^20 points (the size from which the spatial order is automatic): the
+    same sum as the caller's order, up to the FP32 accumulation in LDS."""
+    import torch
+    from spinifel_amd import engine
+    print("case: adjoint_points M=33 2^20 points, spatial order vs caller order", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    n = 1 << 20
+    x = (torch.rand((3, n), generator=g, device="cuda", dtype=torch.float64) * 2 - 1) * 2.5
+    v = torch.randn(n, generator=g, device="cuda", dtype=torch.float32)
+    plan = engine.NufftPlan(33, 1e-6, 0)
+    a = plan.adjoint_points(*x, values=v, spatial=False)
+    b = plan.adjoint_points(*x, values=v)          # 'auto' sorts at this size
+    rel = float((a - b).norm() / a.norm())
+    print("rel-L2 spatial vs caller order", rel, flush=True)
+    assert rel < 1e-6
