@@ -449,7 +449,9 @@ def spatial_order(x, y, z, cell):
     idx = [torch.clamp(((c + 3 * np.pi) / cell).floor().to(torch.int32), 0, 1023)
            for c in (x, y, z)]
     key = _part1by2(idx[0]) | (_part1by2(idx[1]) << 1) | (_part1by2(idx[2]) << 2)
-    return torch.argsort(key)
+    # stable: points of one cube keep the caller's order, so every run adds
+    # the same values in the same order
+    return torch.argsort(key, stable=True)
 
 
 def row_stats(x, n_cols=None, stream=None):
