@@ -36,6 +36,7 @@ Differences a caller can observe:
     reference; ``last`` holds the device result and the iteration count.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -101,9 +102,11 @@ def cg(matvec, b, x0, maxiter, tol=1e-5, callback=None):
 def cg_fused(matvec, b, x0, maxiter, tol=1e-5, callback=None):
     """The same iteration as ``cg`` with libmtip's vector kernels: the step
     lengths stay on the device (rho = r.r of the previous update serves as the
-    next iteration's rho), dot products reduce in a fixed order, and the only
-    host sync per iteration is the stopping test's ||r|| (cupyx's cg syncs
-    there too).  Returns (x, info, iterations)."""
+    next iteration's rho) and dot products reduce in a fixed order.  The
+    stopping test after update k reads ||r_k|| through pinned memory while
+    iteration k + 1's direction and operator application already run, so the
+    GPU does not idle on the host's check; on convergence that look-ahead
+    (which changes neither x nor r) is dropped.  Returns (x, info, iterations)."""
     lib = _lib.load()
     dev = b.device
     st = engine._stream()
@@ -112,6 +115,8 @@ def cg_fused(matvec, b, x0, maxiter, tol=1e-5, callback=None):
     work = torch.empty(int(lib.mtip_dot_workspace_size()) // 8, dtype=torch.float64, device=dev)
     wbytes = work.numel() * 8
     sc = torch.zeros(4, dtype=torch.float64, device=dev)    # rho[0], rho[1], pq, b.b
+    host = torch.empty(1, dtype=torch.float64, pin_memory=True)
+    ready = torch.cuda.Event()
 
     def dot(a, c, slot):
         _lib.check(lib.mtip_dot(P(a), P(c), n, P(work), wbytes,
@@ -132,11 +137,23 @@ def cg_fused(matvec, b, x0, maxiter, tol=1e-5, callback=None):
     it, cur = 0, 0
     dot(r, r, cur)
     resid = float(torch.sqrt(sc[cur]))
-    while it < maxiter:
+    pending = False          # ||r|| of the last update is on its way to `host`
+    while True:
+        if it >= maxiter:
+            if pending:
+                ready.synchronize()
+                resid = math.sqrt(float(host[0]))
+            break
         _lib.check(lib.mtip_cg_direction(P(p), P(r), ptr(cur), ptr(1 - cur), n, int(it == 0), st),
                    "mtip_cg_direction")
         q = matvec(p).contiguous()
         dot(p, q, 2)
+        if pending:
+            ready.synchronize()
+            pending = False
+            resid = math.sqrt(float(host[0]))
+            if resid <= atol:
+                break
         _lib.check(lib.mtip_cg_update(P(x), P(r), P(p), P(q), ptr(cur), ptr(2), n, st),
                    "mtip_cg_update")
         it += 1
@@ -144,9 +161,9 @@ def cg_fused(matvec, b, x0, maxiter, tol=1e-5, callback=None):
             callback(x)
         cur = 1 - cur
         dot(r, r, cur)
-        resid = float(torch.sqrt(sc[cur]))
-        if resid <= atol:
-            break
+        host.copy_(sc[cur:cur + 1], non_blocking=True)
+        ready.record()
+        pending = True
     info = it if (it == maxiter and not resid <= atol) else 0
     return x, info, it
 
