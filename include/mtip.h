@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MTIP_ABI_VERSION 3
+#define MTIP_ABI_VERSION 4
 
 enum {
     MTIP_OK = 0,
@@ -341,8 +341,10 @@ size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision)
 /*
  * host: orientation rows per GEMM tile for `precision`: 0 restores the
  * built-in rule (256 for BF16 and FP8 with N_o, N_i >= 1024, else 128), 128
- * or 256 force a tile.  Process-wide; set it before mtip_match_partial and
- * keep it until the matching mtip_topk_merge.  The 256-row tile needs
+ * or 256 force a tile.  Process-wide; mtip_match_topk reads it once per
+ * call, and mtip_match_partial / mtip_topk_merge take the tile explicitly
+ * (tile_rows) so that a change between them cannot split a call.  The
+ * 256-row tile needs
  * 256 * ld * element bytes < 2^31 (mtip_match_partial reports
  * MTIP_ERR_UNSUPPORTED otherwise).  No counterpart in the reference (the
  * reference has one GEMM path); the project's autotuner sets it from timings.
@@ -380,17 +382,22 @@ int mtip_match_topk(int precision, const void *slices, int64_t ld_s, const float
 /*
  * The two halves of mtip_match_topk, for callers that time or reuse them:
  * mtip_match_partial runs the MFMA GEMM and writes per-(image, orientation
- * tile) top-K packed keys to the workspace (128- or 256-row tiles, chosen
- * from precision and sizes); mtip_topk_merge reduces them to the per-image
- * top-K and must get the same precision and sizes.  Same arguments and
- * rules as mtip_match_topk.
+ * tile) top-K packed keys to the workspace; mtip_topk_merge reduces them to
+ * the per-image top-K and must get the same precision, sizes and tile_rows.
+ * tile_rows: 128 or 256 orientation rows per GEMM tile, or 0 for the value
+ * mtip_gemm_tile gives at the time of the call.  Pass the value
+ * mtip_gemm_tile returned to both calls when another thread may call
+ * mtip_set_gemm_tile in between (0 would then read the new setting in one
+ * call and the old one in the other).  Other arguments and rules as
+ * mtip_match_topk.
  */
 int mtip_match_partial(int precision, const void *slices, int64_t ld_s, const float *slice_scale,
                        const double *slice_sumsq, int64_t N_o, const void *images, int64_t ld_d,
                        const float *image_scale, int64_t N_i, int64_t N_p, const double *alpha,
-                       int K, void *workspace, size_t workspace_bytes, void *stream);
+                       int K, int tile_rows, void *workspace, size_t workspace_bytes,
+                       void *stream);
 int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i, int K,
-                    int32_t *top_idx, float *top_val, void *stream);
+                    int tile_rows, int32_t *top_idx, float *top_val, void *stream);
 
 /*
  * Rescore K candidates per image on FP32 slices/images:

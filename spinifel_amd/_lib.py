@@ -77,8 +77,8 @@ _SIGS = {
     "mtip_set_gemm_tile": (_i32, [_i32, _i32]),
     "mtip_gemm_tile": (_i32, [_i32, _i64, _i64]),
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
-                                  _vp, _i32, _vp, _c.c_size_t, _vp]),
-    "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _vp, _vp, _vp]),
+                                  _vp, _i32, _i32, _vp, _c.c_size_t, _vp]),
+    "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp]),
     "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp,
                             _vp, _vp, _vp, _vp, _vp]),
     "mtip_euclidean_distance": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),

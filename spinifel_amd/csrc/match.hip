@@ -980,9 +980,10 @@ int tile_rows(int precision, int64_t N_o, int64_t N_i)
 template <int PREC, int K>
 void launch_gemm_topk(const void *S, int64_t lds_b, int64_t N_o, const void *D, int64_t ldd_b,
                       int64_t N_i, int64_t kbytes, const float *ssc, const float *dsc,
-                      const double *s2, const double *alpha, uint64_t *part, hipStream_t st)
+                      const double *s2, const double *alpha, int tile, uint64_t *part,
+                      hipStream_t st)
 {
-    if (tile_rows(PREC, N_o, N_i) == BM2) {
+    if (tile == BM2) {
         const int nta = (int)mtip::ceil_div(N_o, BM2), ntb = (int)mtip::ceil_div(N_i, BN2);
         hipLaunchKernelGGL((k_gemm_big<PREC, K>), dim3((unsigned)(nta * ntb)), dim3(512), 0, st,
                            (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i, kbytes,
@@ -1067,41 +1068,47 @@ extern "C" int mtip_gemm_tile(int precision, int64_t N_o, int64_t N_i)
     return tile_rows(precision, N_o, N_i);
 }
 
-extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_s,
-                                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
-                                  const void *images, int64_t ld_d, const float *image_scale,
-                                  int64_t N_i, int64_t N_p, const double *alpha, int K,
-                                  void *workspace, size_t workspace_bytes, void *stream)
+namespace mtip {
+
+// Tile rows the caller asked for (0: the rule and override of tile_rows()).
+int resolve_tile(int precision, int64_t N_o, int64_t N_i, int tile)
 {
-    int rc = check_match_args("mtip_match_partial", precision, slices, ld_s, slice_scale,
-                              slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K);
-    if (rc != MTIP_OK || N_i == 0) return rc;
-    size_t need = mtip_match_workspace_size(N_o, N_i, K, precision);
-    MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM,
-                 "mtip_match_partial: workspace too small");
+    return tile == 0 ? tile_rows(precision, N_o, N_i) : tile;
+}
+
+// Bytes of top-K partial keys for a given tile height.
+size_t partial_bytes(int64_t N_o, int64_t N_i, int K, int tile)
+{
+    return (size_t)(N_i * mtip::ceil_div(N_o, tile) * K) * sizeof(uint64_t);
+}
+
+int partial_impl(const char *fn, int precision, const void *slices, int64_t ld_s,
+                 const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                 const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                 int64_t N_p, const double *alpha, int K, int tile, uint64_t *part,
+                 hipStream_t st)
+{
+    const std::string f(fn);
     const int eb = prec_bytes(precision);
     // the 256-row kernel addresses a tile's rows with 32-bit buffer offsets
-    MTIP_REQUIRE(tile_rows(precision, N_o, N_i) != BM2 ||
-                     (int64_t)BM2 * (ld_s > ld_d ? ld_s : ld_d) * eb < ((int64_t)1 << 31),
+    MTIP_REQUIRE(tile != BM2 || (int64_t)BM2 * (ld_s > ld_d ? ld_s : ld_d) * eb < ((int64_t)1 << 31),
                  MTIP_ERR_UNSUPPORTED,
-                 "mtip_match_partial: rows longer than 8 MiB need the 128-row tile "
-                 "(mtip_set_gemm_tile(precision, 128))");
-    hipStream_t st = (hipStream_t)stream;
-    uint64_t *part = (uint64_t *)workspace;
+                 f + ": rows longer than 8 MiB need the 128-row tile "
+                     "(mtip_set_gemm_tile(precision, 128))");
     const int64_t kbytes = N_p * eb;
 #define MTIP_GEMM_CASE(P)                                                                        \
     switch (K) {                                                                                 \
     case 1: launch_gemm_topk<P, 1>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
             break;                                                                               \
     case 2: launch_gemm_topk<P, 2>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
             break;                                                                               \
     case 4: launch_gemm_topk<P, 4>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, part, st);      \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
             break;                                                                               \
     default: launch_gemm_topk<P, 8>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,    \
-                                    slice_scale, image_scale, slice_sumsq, alpha, part, st);     \
+                                    slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
              break;                                                                              \
     }
     if (precision == MTIP_FP32) { MTIP_GEMM_CASE(MTIP_FP32) }
@@ -1112,16 +1119,10 @@ extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_
     return MTIP_OK;
 }
 
-extern "C" int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i,
-                               int K, int32_t *top_idx, float *top_val, void *stream)
+int merge_impl(const uint64_t *part, int64_t N_o, int64_t N_i, int K, int tile, int32_t *top_idx,
+               float *top_val, hipStream_t st)
 {
-    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_topk_merge: K must be 1, 2, 4 or 8");
-    MTIP_REQUIRE(N_o > 0 && N_i >= 0, MTIP_ERR_ARG, "mtip_topk_merge: bad size");
-    if (N_i == 0) return MTIP_OK;
-    MTIP_REQUIRE(workspace && top_idx, MTIP_ERR_ARG, "mtip_topk_merge: null pointer");
-    hipStream_t st = (hipStream_t)stream;
-    const uint64_t *part = (const uint64_t *)workspace;
-    const int64_t nkeys = mtip::ceil_div(N_o, tile_rows(precision, N_o, N_i)) * K;
+    const int64_t nkeys = mtip::ceil_div(N_o, tile) * K;
     switch (K) {
     case 1: launch_topk_rows<1>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
     case 2: launch_topk_rows<2>(part, nullptr, N_i, nkeys, nkeys, K, top_idx, top_val, st); break;
@@ -1132,6 +1133,71 @@ extern "C" int mtip_topk_merge(int precision, const void *workspace, int64_t N_o
     return MTIP_OK;
 }
 
+int check_tile(const char *fn, int tile)
+{
+    MTIP_REQUIRE(tile == 0 || tile == BM || tile == BM2, MTIP_ERR_ARG,
+                 std::string(fn) + ": tile_rows must be 0, 128 or 256");
+    return MTIP_OK;
+}
+
+int gemm_tile_rows(int precision, int64_t N_o, int64_t N_i)
+{
+    return tile_rows(precision, N_o, N_i);
+}
+
+int check_match(const char *fn, int precision, const void *slices, int64_t ld_s,
+                const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                int64_t N_p, const double *alpha, int K)
+{
+    return check_match_args(fn, precision, slices, ld_s, slice_scale, slice_sumsq, N_o, images,
+                            ld_d, image_scale, N_i, N_p, alpha, K);
+}
+
+}  // namespace mtip
+
+using mtip::check_tile;
+using mtip::merge_impl;
+using mtip::partial_bytes;
+using mtip::partial_impl;
+using mtip::resolve_tile;
+
+extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_s,
+                                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                                  const void *images, int64_t ld_d, const float *image_scale,
+                                  int64_t N_i, int64_t N_p, const double *alpha, int K,
+                                  int tile_rows_arg, void *workspace, size_t workspace_bytes,
+                                  void *stream)
+{
+    int rc = check_match_args("mtip_match_partial", precision, slices, ld_s, slice_scale,
+                              slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K);
+    if (rc == MTIP_OK) rc = check_tile("mtip_match_partial", tile_rows_arg);
+    if (rc != MTIP_OK || N_i == 0) return rc;
+    const int tile = resolve_tile(precision, N_o, N_i, tile_rows_arg);
+    MTIP_REQUIRE(workspace && workspace_bytes >= partial_bytes(N_o, N_i, K, tile), MTIP_ERR_NOMEM,
+                 "mtip_match_partial: workspace too small");
+    return partial_impl("mtip_match_partial", precision, slices, ld_s, slice_scale, slice_sumsq,
+                        N_o, images, ld_d, image_scale, N_i, N_p, alpha, K, tile,
+                        (uint64_t *)workspace, (hipStream_t)stream);
+}
+
+extern "C" int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i,
+                               int K, int tile_rows_arg, int32_t *top_idx, float *top_val,
+                               void *stream)
+{
+    MTIP_REQUIRE(precision == MTIP_FP32 || precision == MTIP_BF16 || precision == MTIP_FP8,
+                 MTIP_ERR_ARG, "mtip_topk_merge: unknown precision");
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG, "mtip_topk_merge: K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(N_o > 0 && N_i >= 0, MTIP_ERR_ARG, "mtip_topk_merge: bad size");
+    int rc = check_tile("mtip_topk_merge", tile_rows_arg);
+    if (rc != MTIP_OK) return rc;
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(workspace && top_idx, MTIP_ERR_ARG, "mtip_topk_merge: null pointer");
+    return merge_impl((const uint64_t *)workspace, N_o, N_i, K,
+                      resolve_tile(precision, N_o, N_i, tile_rows_arg), top_idx, top_val,
+                      (hipStream_t)stream);
+}
+
 extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
                                const float *slice_scale, const double *slice_sumsq, int64_t N_o,
                                const void *images, int64_t ld_d, const float *image_scale,
@@ -1140,11 +1206,19 @@ extern "C" int mtip_match_topk(int precision, const void *slices, int64_t ld_s,
                                float *top_val, void *stream)
 {
     MTIP_REQUIRE(top_idx || N_i == 0, MTIP_ERR_ARG, "mtip_match_topk: null top_idx");
-    int rc = mtip_match_partial(precision, slices, ld_s, slice_scale, slice_sumsq, N_o, images,
-                                ld_d, image_scale, N_i, N_p, alpha, K, workspace, workspace_bytes,
-                                stream);
+    int rc = check_match_args("mtip_match_topk", precision, slices, ld_s, slice_scale,
+                              slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K);
     if (rc != MTIP_OK || N_i == 0) return rc;
-    return mtip_topk_merge(precision, workspace, N_o, N_i, K, top_idx, top_val, stream);
+    // one tile height for both halves, read once
+    const int tile = tile_rows(precision, N_o, N_i);
+    MTIP_REQUIRE(workspace && workspace_bytes >= partial_bytes(N_o, N_i, K, tile), MTIP_ERR_NOMEM,
+                 "mtip_match_topk: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    rc = partial_impl("mtip_match_topk", precision, slices, ld_s, slice_scale, slice_sumsq, N_o,
+                      images, ld_d, image_scale, N_i, N_p, alpha, K, tile, (uint64_t *)workspace,
+                      st);
+    if (rc != MTIP_OK) return rc;
+    return merge_impl((const uint64_t *)workspace, N_o, N_i, K, tile, top_idx, top_val, st);
 }
 
 extern "C" int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *images, int64_t ld_d,

@@ -43,6 +43,24 @@ __device__ __forceinline__ uint64_t make_key(float v, uint32_t idx)
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Host halves of the top-K matching (match.hip), shared with the other
+// matching entry points: argument checks, the GEMM tile height chosen for a
+// call, the partial-key bytes of a tile height, the MFMA GEMM writing
+// per-tile top-K keys, and their merge into the per-image top-K.
+int check_match(const char *fn, int precision, const void *slices, int64_t ld_s,
+                const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                int64_t N_p, const double *alpha, int K);
+int gemm_tile_rows(int precision, int64_t N_o, int64_t N_i);
+size_t partial_bytes(int64_t N_o, int64_t N_i, int K, int tile);
+int partial_impl(const char *fn, int precision, const void *slices, int64_t ld_s,
+                 const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                 const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                 int64_t N_p, const double *alpha, int K, int tile, uint64_t *part,
+                 hipStream_t st);
+int merge_impl(const uint64_t *part, int64_t N_o, int64_t N_i, int K, int tile, int32_t *top_idx,
+               float *top_val, hipStream_t st);
+
 // 64-bit warp shuffle via two 32-bit shuffles.
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask)
 {

@@ -601,14 +601,19 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
     idx = torch.empty((N_i, K), dtype=torch.int32, device=dev)
     val = torch.empty((N_i, K), dtype=torch.float32, device=dev)
     st = _stream(stream)
+    code = _lib.PRECISIONS[precision]
+    # one tile height for both halves, so that a concurrent mtip_set_gemm_tile
+    # (autotuner, another thread) cannot change it between them
+    tile = lib.mtip_gemm_tile(code, N_o, N_i)
+    _lib.check(min(tile, 0), "mtip_gemm_tile")
     with _span(timers, "gemm_" + precision):
-        rc = lib.mtip_match_partial(_lib.PRECISIONS[precision], _ptr(S.data), S.data.shape[1],
+        rc = lib.mtip_match_partial(code, _ptr(S.data), S.data.shape[1],
                                     _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
                                     D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
-                                    _ptr(workspace), workspace.numel(), st)
+                                    tile, _ptr(workspace), workspace.numel(), st)
     _lib.check(rc, "mtip_match_partial")
     with _span(timers, "merge"):
-        rc = lib.mtip_topk_merge(_lib.PRECISIONS[precision], _ptr(workspace), N_o, N_i, K,
+        rc = lib.mtip_topk_merge(code, _ptr(workspace), N_o, N_i, K, tile,
                                  _ptr(idx), _ptr(val), st)
     _lib.check(rc, "mtip_topk_merge")
     return idx, val

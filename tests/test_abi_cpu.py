@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
 
 def test_abi_version_and_workspace_size():
     lib = _lib.load()
-    assert lib.mtip_abi_version() == 3
+    assert lib.mtip_abi_version() == 4
     assert lib.mtip_match_workspace_size(10000, 10000, 8, 1) == 10000 * 79 * 8 * 8
     assert lib.mtip_match_workspace_size(1, 1, 1, 0) == 8
     assert lib.mtip_match_workspace_size(0, 5, 1, 0) == 0
@@ -70,8 +70,8 @@ def test_long_rows_need_the_128_row_tile():
     n, ld = 1024, 1 << 22
     ws = lib.mtip_match_workspace_size(n, n, 8, 1)
     assert lib.mtip_gemm_tile(1, n, n) == 256
-    rc = lib.mtip_match_partial(1, fake, ld, nul, fake, n, fake, ld, nul, n, 16, fake, 8, fake, ws,
-                                nul)
+    rc = lib.mtip_match_partial(1, fake, ld, nul, fake, n, fake, ld, nul, n, 16, fake, 8, 0, fake,
+                                ws, nul)
     assert rc == _lib.MTIP_ERR_UNSUPPORTED and "128-row tile" in _lib.last_error()
 
 
@@ -142,3 +142,21 @@ def test_argument_errors_of_merge_entry_points():
     assert lib.mtip_cg_update(nul, fake, fake, fake, fake, fake, 4, nul) == _lib.MTIP_ERR_ARG
     assert lib.mtip_cg_update(nul, nul, nul, nul, nul, nul, 0, nul) == _lib.MTIP_OK
     assert lib.mtip_cg_direction(fake, fake, fake, nul, 4, 0, nul) == _lib.MTIP_ERR_ARG
+
+
+def test_tile_rows_argument_is_checked():
+    # mtip_match_partial / mtip_topk_merge take the tile height explicitly so
+    # that both halves of one match use the same one
+    lib = _lib.load()
+    nul = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)    # never dereferenced
+    rc = lib.mtip_topk_merge(1, fake, 1000, 10, 8, 192, fake, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "tile_rows" in _lib.last_error()
+    rc = lib.mtip_match_partial(1, fake, 64, nul, fake, 1000, fake, 64, nul, 10, 64, fake, 8, 64,
+                                fake, 1 << 20, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "tile_rows" in _lib.last_error()
+    # an explicit 128-row tile needs the larger partial buffer
+    need128 = 10 * 8 * 8 * 8
+    rc = lib.mtip_match_partial(1, fake, 64, nul, fake, 1000, fake, 64, nul, 10, 64, fake, 8, 128,
+                                fake, need128 - 8, nul)
+    assert rc == _lib.MTIP_ERR_NOMEM and "workspace" in _lib.last_error()
