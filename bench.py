@@ -15,9 +15,13 @@ images (1 photon/px) at random true orientations; synthetic, seeded.
 
 Prints one JSON line.  ``value`` = images/s of the BF16 + rescore form;
 the FP32 (plain) and FP8 forms are timed the same way and reported under
-``forms``.  Accuracy for every form: argmin agreement and relative distance
-error against FP64 distances (numpy) on an image subset; NUFFT slice error
-against the FP64 direct NUDFT on a pixel subsample.
+``forms``.  Accuracy for every form over all images: argmin agreement and
+relative distance error against FP64 distances of every image x orientation
+pair (an FP64 library GEMM on the GPU as the checker, from the same
+slices); agreement with the oracle's FP64 argmin (NUFFT eps 1e-12) on the
+committed fixture tests/golden/m81_16k_px.npz; NUFFT slice error against
+the FP64 direct NUDFT on a pixel subsample.  ``dropin_host_io`` times the
+drop-in ``match`` as the reference calls it, host arrays in and out.
 """
 import argparse
 import gc
@@ -61,6 +65,8 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="rule-based tiles, no autotuning")
     p.add_argument("--no-batching", action="store_true", help="skip the batching form")
     p.add_argument("--no-serving", action="store_true", help="skip the serving form")
+    p.add_argument("--no-dropin", action="store_true",
+                   help="skip timing the drop-in match() with host inputs")
     p.add_argument("--no-merge", action="store_true",
                    help="skip the merge (type-1 + CG) and phasing leg")
     p.add_argument("--merge-only", action="store_true",
@@ -211,38 +217,121 @@ def roofline_entries(W, kern_ms, forms, engine):
     return ent
 
 
-def accuracy(form_res, W, sl, torch, n_sub=256):
-    """Argmin agreement and distance error vs FP64 numpy distances on an image subset."""
-    S = sl.f32[:, :sl.N_p].double().cpu().numpy()
-    D = W["images"].reshape(W["images"].shape[0], -1).astype(np.float64)
-    alpha = D.std() / S.std()
-    sub = np.random.default_rng(123).choice(D.shape[0], min(n_sub, D.shape[0]), replace=False)
-    Ds = D[sub]
-    d2 = (alpha * alpha) * (S * S).sum(1)[:, None] - 2 * alpha * (S @ Ds.T) + (Ds * Ds).sum(1)[None, :]
-    ref_idx = np.argmin(d2, axis=0)
-    srt = np.sort(d2, axis=0)
-    margin = (srt[1] - srt[0]) / srt[0]
+def accuracy(form_res, W, sl, torch, chunk=2000):
+    """Argmin agreement and distance errors of every form over all images,
+    against FP64 distances ||alpha s - d||^2 computed on the GPU (a library
+    FP64 GEMM as the checker) from the same FP32 slices, alpha in FP64."""
+    S = sl.f32[:, :sl.N_p].double()
+    D_all = W["imgset"].f32[:, :sl.N_p]
+    N_i = D_all.shape[0]
+    n_tot = float(N_i * sl.N_p)
+    alpha = float(torch.sqrt(D_all.double().pow(2).sum() / n_tot - (D_all.double().sum() / n_tot) ** 2)
+                  / S.std(unbiased=False))
+    s2 = (S * S).sum(1)
+    ref_idx = torch.empty(N_i, dtype=torch.int64, device=S.device)
+    ref_d2 = torch.empty((N_i, 2), dtype=torch.float64, device=S.device)
+    picks = {f: dict(top=r["topk_idx"][:, 0].long(), idx=r["index"].long()) for f, r in form_res.items()}
+    d2_at = {f: dict(top=torch.empty(N_i, dtype=torch.float64, device=S.device),
+                     idx=torch.empty(N_i, dtype=torch.float64, device=S.device)) for f in form_res}
+    for c0 in range(0, N_i, chunk):
+        Dc = D_all[c0:c0 + chunk].double()
+        d2 = (alpha * alpha) * s2[:, None] - (2 * alpha) * (S @ Dc.T) + (Dc * Dc).sum(1)[None, :]
+        v, i = torch.topk(d2, 2, dim=0, largest=False, sorted=True)
+        ref_idx[c0:c0 + chunk] = torch.argmin(d2, dim=0)    # first index of the minimum
+        ref_d2[c0:c0 + chunk] = v.T
+        cols = torch.arange(Dc.shape[0], device=S.device)
+        for f in form_res:
+            for k in ("top", "idx"):
+                d2_at[f][k][c0:c0 + chunk] = d2[picks[f][k][c0:c0 + chunk], cols]
+        del d2, Dc
+    margin = (ref_d2[:, 1] - ref_d2[:, 0]) / ref_d2[:, 0]
+    c = sl.center[:sl.N_p].double()
     out = {}
-    c = sl.center[:sl.N_p].double().cpu().numpy()
     for f, r in form_res.items():
-        idx = r["index"].long().cpu().numpy()[sub]
-        agree = float((idx == ref_idx).mean())
-        dref = d2[idx, np.arange(len(sub))]
-        v = r["topk_val"].double().cpu().numpy()[sub, 0]
         a = float(r["alpha"].item())
-        est = a * a * (v + ((Ds / a - c[None, :]) ** 2).sum(1))
-        e_gemm = np.abs(est - d2[r["topk_idx"].long().cpu().numpy()[sub, 0], np.arange(len(sub))]) \
-            / d2[r["topk_idx"].long().cpu().numpy()[sub, 0], np.arange(len(sub))]
-        ent = dict(argmin_agreement=agree, gemm_rel_dist_err_max=float(e_gemm.max()),
+        agree = float((picks[f]["idx"] == ref_idx).double().mean())
+        # the GEMM's own estimate of its best candidate's distance:
+        # a^2 (v + ||d / a - c||^2) with v = ||s'||^2 - 2 s'.d' on the centred operands
+        v = r["topk_val"][:, 0].double()
+        dn = torch.empty(N_i, dtype=torch.float64, device=S.device)
+        for c0 in range(0, N_i, chunk):
+            Dc = D_all[c0:c0 + chunk].double()
+            dn[c0:c0 + chunk] = ((Dc / a - c[None, :]) ** 2).sum(1)
+        est = a * a * (v + dn)
+        e_gemm = (est - d2_at[f]["top"]).abs() / d2_at[f]["top"]
+        ent = dict(n_images=int(N_i), argmin_agreement=agree,
+                   n_disagree=int((picks[f]["idx"] != ref_idx).sum()),
+                   gemm_rel_dist_err_max=float(e_gemm.max()),
                    gemm_rel_dist_err_mean=float(e_gemm.mean()))
         if "best_d64" in r:
-            bd = r["best_d64"].cpu().numpy()[sub]
-            e = np.abs(bd - dref) / dref
+            e = (r["best_d64"].double() - ref_d2[:, 0]).abs() / ref_d2[:, 0]
             ent.update(final_rel_dist_err_max=float(e.max()), final_rel_dist_err_mean=float(e.mean()))
         out[f] = ent
-    out["_subset"] = dict(n_images=int(len(sub)), alpha_fp64=float(alpha),
-                          min_rel_margin=float(margin.min()),
-                          n_margin_below_1e6=int((margin < 1e-6).sum()))
+    fp32_bound = out["fp32"]["gemm_rel_dist_err_max"] if "fp32" in out else None
+    out["_reference"] = dict(
+        n_images=int(N_i), alpha_fp64=alpha, min_rel_margin=float(margin.min()),
+        fp32_rel_dist_err_bound=fp32_bound,
+        n_margin_below_fp32_bound=(int((margin < fp32_bound).sum()) if fp32_bound else None),
+        n_margin_below_1e6=int((margin < 1e-6).sum()),
+        checker="FP64 distances of all images x all orientations on the GPU (torch FP64 "
+                "GEMM) from the same FP32 slices")
+    return out
+
+
+GOLDEN_M81 = os.path.join(ROOT, "tests", "golden", "m81_16k_px.npz")
+
+
+def golden_agreement(forms, engine, log):
+    """The drop-in match() of every form on the committed oracle fixture
+    (M=81, 4x32x128 px, 10,000 orientations, 1,000 images; the oracle's FP64
+    argmin at NUFFT eps 1e-12): agreement with the oracle's index."""
+    if not os.path.exists(GOLDEN_M81):
+        return None
+    from spinifel_amd import orientation_matching, settings as st
+    g = dict(np.load(GOLDEN_M81))
+    M = int(g["M"])
+    half = g["ac_half"]
+    ac = np.concatenate([half, half[:-1][::-1]]).reshape(M, M, M)
+    images = g["images"].astype(np.float32)
+    s = st.SpinifelSettings(N_orientations=g["ref_quats"].shape[0],
+                            N_batch_size=int(g["N_batch_size"]), reduced_det_shape=(4, 32, 128))
+    out = dict(fixture="tests/golden/m81_16k_px.npz", n_images=int(images.shape[0]),
+               orientations=int(g["ref_quats"].shape[0]))
+    for f in forms:
+        r = orientation_matching.slicing_and_match(ac, images, g["ppr"], g["pdr"], g["ref_quats"],
+                                                   settings=s, precision=f, return_details=True)
+        out[f] = dict(argmin_agreement=float((r["index"] == g["index"]).mean()),
+                      alpha_rel_err=float(abs(r["alpha"][0] - g["alpha"]) / g["alpha"]))
+        log("golden", f, json.dumps(out[f]))
+    return out
+
+
+def dropin_timing(W, forms, args, torch, log):
+    """The drop-in match() as the reference calls it: host model, host float32
+    images, host quaternions in, host quaternions out (every upload, the
+    slicing, the matching and the download inside the timed call)."""
+    from spinifel_amd import orientation_matching, settings as st
+    p = W["p"]
+    det = (p["panels"], p["rows"], p["cols"])
+    imgs = np.ascontiguousarray(W["images"].reshape((-1,) + det), dtype=np.float32)
+    s = st.SpinifelSettings(N_orientations=p["n_ref"], N_batch_size=1000, reduced_det_shape=det,
+                            nufft_eps=args.eps)
+    out = dict(images_host_bytes=int(imgs.nbytes))
+    for f in forms:
+        def call():
+            return orientation_matching.slicing_and_match(W["ac"], imgs, W["ppr"], W["pdr"],
+                                                          W["ref"], settings=s, precision=f)
+        call()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(3, args.steps // 4)):
+            t0 = time.perf_counter()
+            q = call()
+            ts.append(time.perf_counter() - t0)
+        assert q.shape == (imgs.shape[0], 4)
+        t = float(np.median(ts))
+        out[f] = dict(images_per_s=imgs.shape[0] / t, ms_per_call=1e3 * t, calls=len(ts))
+        log("drop-in", f, json.dumps(out[f]))
     return out
 
 
@@ -695,6 +784,15 @@ def main():
                  for f in forms}
     del W["imgset"], sl, last
     torch.cuda.empty_cache()
+    golden = None
+    if not args.no_check and args.workload == "m81_16k_px":
+        log("oracle fixture")
+        golden = golden_agreement(forms, engine, log)
+    dropin = None
+    if not args.no_dropin:
+        log("drop-in match() with host inputs")
+        dropin = dropin_timing(W, forms, args, torch, log)
+    torch.cuda.empty_cache()
     batching = None if args.no_batching else run_batching(args, torch, engine, synth, log)
     torch.cuda.empty_cache()
     serving_res = None if args.no_serving else run_serving(args, torch, engine, synth, log)
@@ -729,7 +827,9 @@ def main():
         "roofline": rl,
         "kernels": roof,
         "nufft_rel_l2_vs_nudft": nufft_err,
-        "accuracy_subset": acc.get("_subset") if acc else None,
+        "accuracy_reference": acc.get("_reference") if acc else None,
+        "oracle_fixture": golden,
+        "dropin_host_io": dropin,
         "cpu_baseline": cpu,
         "tuning": {"nufft_tiling": W["nufft_tune"], "gemm_tile_rows": gemm_tile,
                    "gemm_timings": gemm_tune},
