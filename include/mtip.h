@@ -400,6 +400,33 @@ int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N
                     int tile_rows, int32_t *top_idx, float *top_val, void *stream);
 
 /*
+ * Bytes of the device scratch mtip_match_topk_static uses (32 MiB), one
+ * copy per GPU the library runs on, held until the process ends.
+ */
+#define MTIP_STATIC_SCRATCH_BYTES (32u << 20)
+
+/*
+ * mtip_match_topk without a workspace argument: the top-K partial keys go
+ * to a device array inside the library (MTIP_STATIC_SCRATCH_BYTES per GPU),
+ * so the call allocates no device memory and calls no allocating routine.
+ * No counterpart in the reference (its KNN allocates and frees device
+ * buffers on every call, orientation_matching_sp.cu:625-627, :677); it is
+ * for callers that cannot allocate: code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.
+ * Limit: N_i * ceil(N_o / tile) * K * 8 bytes <= MTIP_STATIC_SCRATCH_BYTES
+ * with tile = mtip_gemm_tile(precision, N_o, N_i); larger calls return
+ * MTIP_ERR_ARG (split the images into several calls).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race and must be serialised by the caller.  Same arguments
+ * and results as mtip_match_topk otherwise.
+ */
+int mtip_match_topk_static(int precision, const void *slices, int64_t ld_s,
+                           const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                           const void *images, int64_t ld_d, const float *image_scale,
+                           int64_t N_i, int64_t N_p, const double *alpha, int K,
+                           int32_t *top_idx, float *top_val, void *stream);
+
+/*
  * Rescore K candidates per image on FP32 slices/images:
  *   d(n, i) = sum_p (alpha s_n[p] - d_i[p])^2
  * with alpha[0] for every image, or alpha[i] per image when

@@ -79,6 +79,8 @@ _SIGS = {
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                                   _vp, _i32, _i32, _vp, _c.c_size_t, _vp]),
     "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp]),
+    "mtip_match_topk_static": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64,
+                                      _i64, _vp, _i32, _vp, _vp, _vp]),
     "mtip_rescore": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _vp, _i32, _vp, _i32, _vp,
                             _vp, _vp, _vp, _vp, _vp]),
     "mtip_euclidean_distance": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),

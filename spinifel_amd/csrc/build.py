@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libmtip.so")
-SOURCES = ["common.hip", "nufft.hip", "adjoint.hip", "match.hip", "hier.hip", "phasing.hip", "prep.hip", "merge.hip"]
+SOURCES = ["common.hip", "nufft.hip", "adjoint.hip", "match.hip", "hier.hip", "phasing.hip", "prep.hip", "merge.hip", "static_match.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
