@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_interp(GridView gv, const double *__res
 
 // Taps of one lane's element pair from the staged box: rows b of plane a at
 // compile-time offsets b * S, planes at runtime offsets a * plane.
-constexpr int kPlanesInFlight = 2;
+constexpr int kPlanesInFlight = 3;
 
 template <int W, int S>
 __device__ __forceinline__ void taps_pair_lds(const float *__restrict__ box, int base, int plane,
@@ -340,9 +340,13 @@ __device__ __forceinline__ void taps_pair_lds(const float *__restrict__ box, int
         }
         a0 = fmaf(wa[a], t0, a0);
         a1 = fmaf(wa[a], t1, a1);
-        // keep the reads of at most kPlanesInFlight planes ahead of their FMAs:
-        // hoisting all W^2 reads costs 2 W^2 registers and occupancy
-        if ((a + 1) % kPlanesInFlight == 0) __builtin_amdgcn_sched_barrier(0);
+        // issue the reads kPlanesInFlight planes at a time, each group's
+        // reads ahead of the previous group's FMAs: the compiler otherwise
+        // pairs and hoists nearly all W^2 reads above the first FMA (a
+        // scheduling barrier does not stop the load pairing, a compiler
+        // memory barrier does).  At M = 81, 10k x 16k: 8.02 ms with every read
+        // hoisted, 8.31 ms one plane at a time, 7.87 ms three planes at a time
+        if ((a + 1) % kPlanesInFlight == 0) asm volatile("" ::: "memory");
     }
     r0 = a0;
     r1 = a1;

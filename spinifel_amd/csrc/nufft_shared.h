@@ -123,7 +123,12 @@ struct PairTaps {
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+    // mov_dpp has no "old" operand: lanes whose source is outside the row are
+    // left undefined instead of taking a value that first has to be written
+    // (update_dpp(0, ...) costs one v_mov per use).  Callers read only lanes
+    // with a valid source: quad_perm sources are always valid, and after
+    // row_shl the group sums are read in lane 0 of each group.
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
 }
 
 template <int G>
@@ -159,11 +164,18 @@ __device__ __forceinline__ void gather_weights(const float (&own)[PairTaps<W>::N
     }
 }
 
+// Minimum over the wave, in every lane: DPP within each row of 16 lanes
+// (VALU; the xor shuffles this replaces were LDS permutes, and the LDS is the
+// slicing kernel's binding unit), then the four row results by readlane.
 __device__ __forceinline__ int wave_min_i(int v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
-    return v;
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false));    // quad_perm [1,0,3,2]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false));    // quad_perm [2,3,0,1]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x124, 0xf, 0xf, false));   // row_ror:4
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false));   // row_ror:8
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return min(min(r0, r1), min(r2, r3));
 }
 
 // Window corner as separate integer coordinates (padded-grid indices).
