@@ -541,17 +541,26 @@ def residual(x, precision, n_cols=None, mult=None, invert=False, center=None, st
     return Operand(out, sc, ss)
 
 
-def col_mean(x, n_cols=None, stream=None):
-    """Column means of (rows, ld) float32 rows -> (ld,) float32 (zeros past n_cols)."""
+def col_mean(x, n_cols=None, stream=None, row_step=1):
+    """Column means of (rows, ld) float32 rows -> (ld,) float32 (zeros past n_cols),
+    over every row_step-th row (rows 0, row_step, 2 row_step, ...)."""
     lib = _lib.load()
     rows, ld = x.shape
     cols = ld if n_cols is None else n_cols
+    row_step = max(1, int(row_step))
     out = torch.zeros(ld, dtype=torch.float32, device=x.device)
     nb = int(lib.mtip_col_mean_workspace_size(cols))
     work = torch.empty(max(nb, 8), dtype=torch.uint8, device=x.device)
-    _lib.check(lib.mtip_col_mean(_ptr(x), rows, cols, ld, _ptr(work), work.numel(), _ptr(out),
-                                 _stream(stream)), "mtip_col_mean")
+    _lib.check(lib.mtip_col_mean(_ptr(x), (rows + row_step - 1) // row_step, cols, ld * row_step,
+                                 _ptr(work), work.numel(), _ptr(out), _stream(stream)),
+               "mtip_col_mean")
     return out
+
+
+# rows the centre of a slice set is averaged over (at least): any common vector
+# leaves the distances unchanged, and the mean of about a thousand orientations
+# already removes the orientation-independent part of the pattern
+_CENTER_ROWS = 1024
 
 
 class ImageSet:
@@ -698,9 +707,14 @@ class SliceSet:
 
     @property
     def center(self):
-        """Per-pixel mean slice (float32, ld), the centre both operands subtract."""
+        """Per-pixel mean slice (float32, ld), the centre both operands subtract:
+        the mean of every k-th orientation's slice, k = N_o // 1024 (all of
+        them below 2048 orientations).  Distances are invariant to the choice
+        of centre; it only has to remove the orientation-independent part of
+        the pattern before the cast to bf16 / e4m3."""
         if self._center is None:
-            self._center = col_mean(self.f32, self.N_p)
+            self._center = col_mean(self.f32, self.N_p,
+                                    row_step=max(1, self.N_o // _CENTER_ROWS))
         return self._center
 
     @property
