@@ -461,7 +461,9 @@ int mtip_euclidean_distance(const float *ref, const float *data, int64_t numRef,
 /*
  * Indices of the totalNN smallest entries of each row of euDist
  * (numData, numRef), ascending, lowest index first on ties: idx device
- * int32 (numData, totalNN).  totalNN in 1..8.
+ * int32 (numData, totalNN); entries past numRef are -1.  Any totalNN >= 1:
+ * up to 8 by one register top-K pass per row, beyond that by a radix
+ * select and an LDS sort per row, 2048 indices at a time.
  * Replaces cudaHeapSort(euDist, numDataImgs, numRefImgs, totalPixels,
  * totalNN, deviceId) (orientation_matching_sp.cu:682-729).
  */

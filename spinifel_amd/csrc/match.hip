@@ -100,10 +100,13 @@ constexpr int kXcds = 8;
 
 __device__ __forceinline__ void tile_of(int bid, int nblocks, int nta, int ntb, int &ta, int &tb)
 {
-    const int per = (nblocks + kXcds - 1) / kXcds;
+    // workgroup bid runs on XCD bid % 8 as its (bid / 8)-th; XCD x runs q + 1
+    // workgroups when x < rem, else q (nblocks = 8 q + rem), and takes the
+    // contiguous tile range that starts at x q + min(x, rem): a bijection of
+    // [0, nblocks) for any nblocks
+    const int q = nblocks / kXcds, rem = nblocks % kXcds;
     const int xcd = bid % kXcds, local = bid / kXcds;
-    int t = xcd * per + local;
-    if (t >= nblocks) t = bid;        // uneven split: fall back to the launch order
+    const int t = xcd * q + min(xcd, rem) + local;
     constexpr int GM = 8;
     const int in_group = GM * ntb;
     const int group = t / in_group;
@@ -996,6 +999,103 @@ void launch_gemm_topk(const void *S, int64_t lds_b, int64_t N_o, const void *D, 
     }
 }
 
+// Indices of the K smallest entries of each row for any K: one workgroup per
+// row, output in chunks of up to kSelChunk keys.  Keys are (order-preserving
+// float bits << 32 | column), so they are unique and ties go to the lowest
+// column.  Per chunk, the rank-r key T among the keys above the previous
+// chunk's last key L is found by a radix select (eight passes of 8-bit
+// digits, each a histogram of the row in LDS); the keys in (L, T] are then
+// gathered, bitonic-sorted in LDS and written.  Columns past the row give -1.
+constexpr int kSelChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_select_rows(const float *__restrict__ vals, int64_t n_rows,
+                                                     int64_t n, int64_t ld, int64_t K,
+                                                     int32_t *__restrict__ idx)
+{
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t buf[kSelChunk];
+    __shared__ uint64_t s_prefix;
+    __shared__ int64_t s_rank;
+    __shared__ uint32_t s_count;
+    const int tid = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    if (row >= n_rows) return;
+    const float *v = vals + row * ld;
+    int32_t *out = idx + row * K;
+    bool have_lo = false;
+    uint64_t lo = 0;                    // last key written (exclusive lower bound)
+    for (int64_t done = 0; done < K;) {
+        // keys above lo
+        if (tid == 0) s_count = 0;
+        __syncthreads();
+        uint32_t c = 0;
+        for (int64_t e = tid; e < n; e += 256) {
+            const uint64_t k = mtip::make_key(v[e], (uint32_t)e);
+            c += (!have_lo || k > lo) ? 1u : 0u;
+        }
+        atomicAdd(&s_count, c);
+        __syncthreads();
+        const int64_t avail = s_count;
+        const int64_t want = min((int64_t)kSelChunk, K - done);
+        const int64_t take = min(want, avail);
+        if (take == 0) {
+            for (int64_t r = done + tid; r < K; r += 256) out[r] = -1;
+            return;
+        }
+        // radix select of the take-th smallest key above lo
+        if (tid == 0) { s_prefix = 0; s_rank = take; }
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 56 - 8 * pass;
+            hist[tid] = 0;
+            __syncthreads();
+            const uint64_t prefix = s_prefix;
+            for (int64_t e = tid; e < n; e += 256) {
+                const uint64_t k = mtip::make_key(v[e], (uint32_t)e);
+                if ((have_lo && k <= lo) || (pass > 0 && (k >> (shift + 8)) != prefix)) continue;
+                atomicAdd(&hist[(k >> shift) & 255], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int64_t r = s_rank;
+                int d = 0;
+                for (; d < 255 && (int64_t)hist[d] < r; ++d) r -= hist[d];
+                s_prefix = (prefix << 8) | (uint64_t)d;
+                s_rank = r;
+            }
+            __syncthreads();
+        }
+        const uint64_t hi = s_prefix;   // the take-th key: keys in (lo, hi] are this chunk
+        if (tid == 0) s_count = 0;
+        for (int i = tid; i < kSelChunk; i += 256) buf[i] = KEY_NONE;
+        __syncthreads();
+        for (int64_t e = tid; e < n; e += 256) {
+            const uint64_t k = mtip::make_key(v[e], (uint32_t)e);
+            if ((!have_lo || k > lo) && k <= hi) buf[atomicAdd(&s_count, 1u)] = k;
+        }
+        __syncthreads();
+        int len = 1;
+        while (len < take) len <<= 1;
+        for (int size = 2; size <= len; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < len; i += 256) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const bool up = (i & size) == 0;
+                        const uint64_t a = buf[i], b = buf[j];
+                        if ((a > b) == up) { buf[i] = b; buf[j] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int64_t r = tid; r < take; r += 256) out[done + r] = (int32_t)(uint32_t)buf[r];
+        __syncthreads();
+        lo = hi;
+        have_lo = true;
+        done += take;
+    }
+}
+
 template <int K>
 void launch_topk_rows(const uint64_t *keys, const float *vals, int64_t rows, int64_t n, int64_t ld,
                       int kout, int32_t *idx, float *val, hipStream_t st)
@@ -1362,12 +1462,16 @@ extern "C" int mtip_heap_sort(const float *euDist, int64_t numData, int64_t numR
                               int32_t *idx, void *stream)
 {
     MTIP_REQUIRE(numData >= 0 && numRef >= 0, MTIP_ERR_ARG, "mtip_heap_sort: negative size");
-    MTIP_REQUIRE(totalNN >= 1 && totalNN <= 8, MTIP_ERR_ARG, "mtip_heap_sort: totalNN must be 1..8");
+    MTIP_REQUIRE(totalNN >= 1, MTIP_ERR_ARG, "mtip_heap_sort: totalNN must be at least 1");
     if (numData == 0) return MTIP_OK;
     MTIP_REQUIRE(numRef <= 0x7fffffff, MTIP_ERR_ARG, "mtip_heap_sort: numRef too large");
+    MTIP_REQUIRE(numData <= 0x7fffffff, MTIP_ERR_ARG, "mtip_heap_sort: numData too large");
     MTIP_REQUIRE(euDist && idx, MTIP_ERR_ARG, "mtip_heap_sort: null pointer");
     hipStream_t st = (hipStream_t)stream;
-    if (totalNN == 1)
+    if (totalNN > 8) {
+        hipLaunchKernelGGL(k_select_rows, dim3((unsigned)numData), dim3(256), 0, st, euDist, numData,
+                           numRef, numRef, (int64_t)totalNN, idx);
+    } else if (totalNN == 1)
         launch_topk_rows<1>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);
     else if (totalNN == 2)
         launch_topk_rows<2>(nullptr, euDist, numData, numRef, numRef, totalNN, idx, nullptr, st);

@@ -70,3 +70,21 @@ def test_nearest_neighbor_reference_arguments(data):
     print("knn: nearest_neighbor 512 x 1024 x 16384", flush=True)
     index = nn.nearest_neighbor(model_slices, slices, 128)
     np.testing.assert_array_equal(index, matching.nearest_neighbor(model_slices, slices))
+
+
+@pytest.mark.parametrize("totalNN", [1, 3, 8, 9, 100, 2500, 3000])
+def test_heap_sort_any_total_nn(totalNN):
+    """mtip_heap_sort for any totalNN (cudaHeapSort, orientation_matching_sp.cu:682,
+    takes any count): the totalNN smallest per row, ascending, lowest index on
+    ties (entries repeat values on purpose), -1 past the row's length."""
+    import torch
+    from spinifel_amd import engine
+    rng = np.random.default_rng(totalNN)
+    n_ref = 2600
+    eu = rng.integers(0, 500, (37, n_ref)).astype(np.float32)    # many ties
+    eu[5, :] = 7.0                                                # a row of equal values
+    got = engine.heap_sort(torch.as_tensor(eu).cuda(), totalNN).cpu().numpy()
+    want = np.argsort(eu, axis=1, kind="stable")[:, :totalNN]
+    if totalNN > n_ref:
+        want = np.concatenate([want, np.full((37, totalNN - n_ref), -1)], axis=1)
+    np.testing.assert_array_equal(got, want)

@@ -130,6 +130,49 @@ def test_topk_256_row_tile(dev, precision):
         assert same >= 0.99
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_topk_tiles_agree_after_rescoring(dev, precision):
+    # the 128- and 256-row kernels accumulate the cross term in different
+    # orders, so their FP32 keys can differ in the last bits and reorder
+    # near-equal candidates (hence serving fixes the tile per model, and the
+    # drop-in's rule depends on the sizes only); their candidate lists still
+    # agree to rounding and the FP64 rescore picks the FP64 argmin with both.
+    # 6 x 5 = 30 tiles of 256: not a multiple of the 8 XCDs, the case the
+    # 256-row kernel's tile order once mapped onto 29 distinct tiles
+    import torch
+    from spinifel_amd import engine
+    N_o, N_i, N_p = 1300, 1100, 2048
+    print(f"case: topk tile 128 vs 256 {precision} {(N_o, N_i, N_p)}", flush=True)
+    rng = np.random.default_rng(12)
+    S = rng.gamma(2.0, 1.0, (N_o, N_p))
+    Dm = rng.poisson(S[rng.integers(0, N_o, N_i)] * 1.1).astype(np.float64)
+    ss = _sliceset(S, dev)
+    im = engine.ImageSet(Dm, dev)
+    a = torch.tensor([1.1], dtype=torch.float64, device=dev)
+    Sop = ss.operand(precision, True)
+    Dop = im.operand(precision, a, ss.center)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    best = {}
+    sets = {}
+    for t in (128, 256):
+        idx, val = engine.topk(precision, Sop, Sop.sumsq, N_o, Dop, N_i, N_p, one, 8, tile=t)
+        r = engine.rescore(ss.f32, im.f32, N_p, a, idx)
+        best[t] = r["best"].cpu().numpy()
+        sets[t] = idx.cpu().numpy()
+    ref_idx, ref_d = matching.topk_squared(1.1 * S.astype(np.float32).astype(np.float64),
+                                           Dm.astype(np.float32).astype(np.float64), 8)
+    margin = (ref_d[:, 1] - ref_d[:, 0]) / ref_d[:, 0]
+    for t in (128, 256):
+        bad = np.flatnonzero(best[t] != ref_idx[:, 0])
+        inset = np.mean([ref_idx[i, 0] in sets[t][i] for i in range(N_i)])
+        print(f"tile {t}: {len(bad)} images off the FP64 argmin {bad[:10].tolist()}, margins "
+              f"{margin[bad][:10].tolist()}, FP64 best in top-8 for {inset:.4f}", flush=True)
+    for t in (128, 256):
+        np.testing.assert_array_equal(best[t], ref_idx[:, 0])
+    overlap = np.mean([len(set(x) & set(y)) / 8 for x, y in zip(sets[128], sets[256])])
+    assert overlap >= 0.99
+
+
 def test_ties_lowest_index(dev):
     import torch
     from spinifel_amd import engine
