@@ -449,8 +449,11 @@ def spatial_order(x, y, z, cell):
     idx = [torch.clamp(((c + 3 * np.pi) / cell).floor().to(torch.int32), 0, 1023)
            for c in (x, y, z)]
     key = _part1by2(idx[0]) | (_part1by2(idx[1]) << 1) | (_part1by2(idx[2]) << 2)
-    # stable: points of one cube keep the caller's order, so every run adds
-    # the same values in the same order
+    # stable: points of one cube keep the caller's order, so each 128-point
+    # run holds the same points in the same order on every call and fills its
+    # LDS box identically; the runs' boxes still reach the FP64 grid through
+    # global atomics, whose order varies, so the grid is reproducible to
+    # rounding, not bitwise
     return torch.argsort(key, stable=True)
 
 
@@ -597,10 +600,12 @@ def workspace_bytes(N_o, N_i, K, precision):
 
 
 def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stream=None,
-         timers=None):
+         timers=None, tile=None):
     """Top-K orientations per image by v = alpha^2 ||s||^2 - 2 alpha s.d.
 
     S, D: Operand; returns (idx int32 (N_i, K), val float32 (N_i, K)).
+    tile: GEMM tile rows (128 or 256); None takes mtip_gemm_tile's choice for
+    these sizes.
     """
     lib = _lib.load()
     dev = D.data.device
@@ -611,10 +616,12 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
     val = torch.empty((N_i, K), dtype=torch.float32, device=dev)
     st = _stream(stream)
     code = _lib.PRECISIONS[precision]
-    # one tile height for both halves, so that a concurrent mtip_set_gemm_tile
-    # (autotuner, another thread) cannot change it between them
-    tile = lib.mtip_gemm_tile(code, N_o, N_i)
-    _lib.check(min(tile, 0), "mtip_gemm_tile")
+    if tile is None:
+        # one tile height for both halves, so that a concurrent
+        # mtip_set_gemm_tile (autotuner, another thread) cannot change it
+        # between them
+        tile = lib.mtip_gemm_tile(code, N_o, N_i)
+        _lib.check(min(tile, 0), "mtip_gemm_tile")
     with _span(timers, "gemm_" + precision):
         rc = lib.mtip_match_partial(code, _ptr(S.data), S.data.shape[1],
                                     _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),

@@ -6,13 +6,20 @@ GEMM format, bf16 by default).  Requests (a batch of images tagged with a
 model) are queued; a scheduler thread admits them onto ``n_slots`` HIP
 streams.  Whenever a slot is free it takes the oldest pending request and
 every other pending request for the same model (up to ``max_batch_images``),
-provided the device has enough free HBM for the batch (``mtip_mem_info``);
-requests that arrive while others run are therefore batched as soon as a
-slot frees up, and their work overlaps with the other slots on the device.
+provided the device has enough free HBM for the batch (``mtip_mem_info``
+plus the blocks PyTorch's caching allocator holds free); requests that
+arrive while others run are therefore batched as soon as a slot frees up,
+and their work overlaps with the other slots on the device.  A batch that
+cannot be admitted while every slot is idle, even after the allocator's
+cache is released, fails its requests with MemoryError after
+``admit_timeout_s`` instead of waiting forever.
 
 Each request is scaled by its own std ratio (``mtip_segment_scaling_ratio``)
 and every per-image quantity (cross term, top-K keys, FP64 rescore) depends
-only on that image, so a request's result is the same alone and in any mix.
+only on that image; the GEMM tile height is fixed per model (from its
+orientation count, not the batch size), so the same kernel scores a request
+whatever shares its batch, and a request's result is the same alone and in
+any mix, bitwise.
 """
 import ctypes
 import threading
@@ -35,6 +42,7 @@ class ServedModel:
     operand: engine.Operand
     N_p: int
     det_shape: tuple
+    tile: int                     # GEMM tile rows used for every batch of this model
 
 
 @dataclass(eq=False)
@@ -67,7 +75,7 @@ def free_hbm(device=0):
 
 class Server:
     def __init__(self, precision="bf16", K=8, n_slots=4, max_batch_images=4096,
-                 mem_reserve_bytes=8 << 30, device=0, eps=1e-6):
+                 mem_reserve_bytes=8 << 30, device=0, eps=1e-6, admit_timeout_s=30.0):
         engine.require_gpu()
         self.precision = precision
         self.K = K
@@ -75,6 +83,8 @@ class Server:
         self.n_slots = n_slots
         self.max_batch_images = max_batch_images
         self.mem_reserve = mem_reserve_bytes
+        self.admit_timeout_s = admit_timeout_s
+        self._blocked_since = None     # when admission first failed with every slot idle
         self.eps = eps
         self.models = {}
         self._plans = {}
@@ -113,7 +123,11 @@ class Server:
         if int(sl.bad.item()) > 0:
             raise AssertionError("NU points outside [-3pi, 3pi)")
         op = sl.operand(self.precision, True)
-        self.models[name] = ServedModel(name, np.asarray(ref_quats), sl, op, sl.N_p, ppr.shape[1:])
+        # tile rows from the orientation count alone: the 256-row tile for the
+        # narrow formats once there are enough orientations to fill it
+        tile = 256 if (self.precision != "fp32" and sl.N_o >= 1024) else 128
+        self.models[name] = ServedModel(name, np.asarray(ref_quats), sl, op, sl.N_p, ppr.shape[1:],
+                                        tile)
         for slot in range(self.n_slots + 1):     # pinned staging allocated up front
             self._stage_buffer(slot, self.max_batch_images, sl.N_p)
         return self.models[name]
@@ -180,7 +194,7 @@ class Server:
             D = imgs.operand(self.precision, alpha, m.slices.center, stream)
             one = engine._one(self.device)
             idx, val = engine.topk(self.precision, m.operand, m.operand.sumsq, m.slices.N_o, D,
-                                   imgs.N_i, imgs.N_p, one, self.K, None, stream)
+                                   imgs.N_i, imgs.N_p, one, self.K, None, stream, tile=m.tile)
             rs = engine.rescore(m.slices.f32, imgs.f32, imgs.N_p, alpha, idx, stream)
             best = rs["best"].to("cpu", non_blocking=True)
             bd = rs["best_d64"].to("cpu", non_blocking=True)
@@ -226,12 +240,40 @@ class Server:
                 break
             batch.append(r)
             n += len(r.images)
-        free, _ = free_hbm(self.device.index or 0)
-        if free - self.mem_reserve < self._batch_bytes(n, m.N_p):
-            return None                      # wait for memory to free up
+        need = self._batch_bytes(n, m.N_p)
+        if self._available_bytes() - self.mem_reserve < need:
+            if all(b is None for b in self._busy):
+                # nothing running will free memory: give the allocator's cache
+                # back, and fail the batch if it still does not fit in time
+                torch.cuda.empty_cache()
+                if self._available_bytes() - self.mem_reserve >= need:
+                    self._blocked_since = None
+                else:
+                    now = time.perf_counter()
+                    if self._blocked_since is None:
+                        self._blocked_since = now
+                    elif now - self._blocked_since > self.admit_timeout_s:
+                        self._blocked_since = None
+                        err = MemoryError(f"batch of {n} images needs {need} bytes of HBM "
+                                          f"beyond the {self.mem_reserve}-byte reserve; "
+                                          "none came free")
+                        for r in batch:
+                            self._queue.remove(r)
+                            r.future.set_exception(err)
+                    return None
+            else:
+                return None                  # wait for running batches to free memory
+        self._blocked_since = None
         for r in batch:
             self._queue.remove(r)
         return batch
+
+    def _available_bytes(self):
+        """Free HBM plus the blocks PyTorch's caching allocator holds unused
+        (freed tensors stay reserved by the allocator and out of hipMemGetInfo)."""
+        free, _ = free_hbm(self.device.index or 0)
+        cached = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+        return free + max(0, cached)
 
     def _loop(self):
         try:

@@ -85,3 +85,70 @@ def test_admission_waits_for_memory(server):
     res = fut.result(timeout=60)
     srv.stop()
     assert len(res.index) == 10
+
+
+def test_admission_gives_up_when_nothing_can_free_memory(server):
+    """A batch that cannot fit while every slot is idle fails with MemoryError
+    after the admission timeout (released cache included) instead of waiting
+    forever; the server keeps serving afterwards."""
+    from spinifel_amd import serving
+    srv, models = server
+    free, _ = serving.free_hbm(0)
+    old_res, old_to = srv.mem_reserve, srv.admit_timeout_s
+    srv.mem_reserve = free + (1 << 40)
+    srv.admit_timeout_s = 0.2
+    srv.start()
+    try:
+        fut = srv.submit("a", models["a"]["imgs"][:10])
+        with pytest.raises(MemoryError):
+            fut.result(timeout=30)
+        srv.mem_reserve = old_res
+        res = srv.submit("a", models["a"]["imgs"][:10]).result(timeout=60)
+        assert len(res.index) == 10
+    finally:
+        srv.mem_reserve, srv.admit_timeout_s = old_res, old_to
+        srv.stop()
+
+
+def test_config4_models_alone_equal_in_mix():
+    """The serving form at its configured scale: four resident models (M = 81,
+    81, 121, 121 with 10,000 / 100,000 / 10,000 / 100,000 orientations, 4 x 32 x
+    128 px, bf16 reference slices); a sample of Poisson-burst requests (sizes
+    in [1, 512]) gives bitwise the same orientations and FP64 distances alone
+    and in the mix."""
+    import torch
+    from spinifel_amd import engine, serving
+    ppr, pdr = synth.detector(4, 32, 128)
+    E = float(pdr.max())
+    srv = serving.Server(precision="bf16", K=8, n_slots=4, max_batch_images=4096)
+    spec = [("m81_10k", 81, 800, 10000), ("m81_100k", 81, 800, 100000),
+            ("m121_10k", 121, 2700, 10000), ("m121_100k", 121, 2700, 100000)]
+    pools = {}
+    for k, (name, M, atoms, n_ref) in enumerate(spec):
+        print(f"serving config 4: model {name}", flush=True)
+        ac = synth.make_model(M, atoms, k)
+        ref = synth.random_quaternions(n_ref, synth.rng_for(k, 9))
+        srv.add_model(name, ac, ref, ppr, pdr)
+        m = srv.models[name]
+        assert m.tile == 256
+        true = synth.random_quaternions(600, synth.rng_for(k, 10))
+        plan = srv._plans[(M, srv.eps)]
+        q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(srv.device)
+        sl = plan.slices(torch.as_tensor(true).to(srv.device), q, np.pi / E,
+                         order=engine.auto_tiling(plan, ppr.shape[1:]))["s32"][:, :q.shape[1]]
+        imgs = synth.poisson_images(sl.double().cpu().numpy(), 1.0, synth.rng_for(k, 11))
+        pools[name] = imgs
+    rng = synth.rng_for(4, 4)
+    sched = serving.poisson_bursts(120.0, 0.2, 512, [s[0] for s in spec], rng)
+    reqs = [(name, pools[name][rng.integers(0, 600, n)]) for _, name, n in sched[:16]]
+    print(f"serving config 4: {len(reqs)} requests, {sum(len(i) for _, i in reqs)} images",
+          flush=True)
+    alone = [srv.match_now(name, imgs) for name, imgs in reqs]
+    srv.start()
+    futs = [srv.submit(name, imgs) for name, imgs in reqs]
+    mixed = [f.result(timeout=300) for f in futs]
+    srv.stop()
+    assert max(r.batch_images for r in mixed) > max(len(i) for _, i in reqs), "no batching happened"
+    for a, m in zip(alone, mixed):
+        np.testing.assert_array_equal(a.index, m.index)
+        np.testing.assert_array_equal(a.dist64, m.dist64)
