@@ -298,6 +298,8 @@ def golden_agreement(forms, engine, log):
     out = dict(fixture="tests/golden/m81_16k_px.npz", n_images=int(images.shape[0]),
                orientations=int(g["ref_quats"].shape[0]))
     for f in forms:
+        # the FP32 entry is the plain form (use_single_prec: K = 1, no rescoring)
+        s.use_single_prec = f == "fp32"
         r = orientation_matching.slicing_and_match(ac, images, g["ppr"], g["pdr"], g["ref_quats"],
                                                    settings=s, precision=f, return_details=True)
         out[f] = dict(argmin_agreement=float((r["index"] == g["index"]).mean()),
@@ -309,7 +311,9 @@ def golden_agreement(forms, engine, log):
 def dropin_timing(W, forms, args, torch, log):
     """The drop-in match() as the reference calls it: host model, host float32
     images, host quaternions in, host quaternions out (every upload, the
-    slicing, the matching and the download inside the timed call)."""
+    slicing, the matching and the download inside the timed call), with the
+    default settings (use_single_prec False: the FP32 form rescores its top-4
+    in FP64, the reference's float64 distance semantics)."""
     from spinifel_amd import orientation_matching, settings as st
     p = W["p"]
     det = (p["panels"], p["rows"], p["cols"])

@@ -153,6 +153,9 @@ class SNM:
         self.pixel_position_reciprocal = pixel_position_reciprocal
         self.nufft = nufft
         self.precision = precision or getattr(settings, "precision", "fp32")
+        # FP64 rescoring unless use_single_prec (the reference's f_type rule)
+        from .orientation_matching import resolve_form
+        _, self.K, self.rescore = resolve_form(settings, self.precision)
         self.images = None
         if self.N_slices:
             self.images = engine.ImageSet(slices_.reshape(self.N_slices, self.N_pixels),
@@ -181,7 +184,8 @@ class SNM:
         scaled = engine.residual(sl.f32, "fp32", sl.N_p, mult=ratio, invert=True)
         model = engine.SliceSet.from_rows(scaled.data, sl.N_p)
         one = engine._one(self.nufft.device)
-        r = engine.match_resident(model, self.images, self.precision, alpha=one)
+        r = engine.match_resident(model, self.images, self.precision, K=self.K,
+                                  rescore_fp64=self.rescore, alpha=one)
         index = r["index"].long().cpu().numpy()
         self.last = dict(index=index, mult=(1.0 / ratio).cpu().numpy(), result=r)
         return self.nufft.ref_orientations[index]

@@ -30,17 +30,26 @@ from .autocorrelation import get_plan
 from .settings import settings as _settings
 
 
-def _resolve_form(s, precision, topk, rescore):
-    precision = precision or s.precision
+def resolve_form(s, precision=None, topk=None, rescore=None):
+    """(precision, K, rescore) of a match from the arguments and the settings.
+
+    BF16 / FP8: top-8 rescored in FP32 then FP64.  FP32: with
+    ``use_single_prec`` the plain FP32 argmin (K = 1, no rescoring); without
+    it (the reference's default, float64 distances,
+    spinifel/sequential/orientation_matching.py:32-37) the top-4 of the FP32
+    cross term are rescored in FP64, so the index is the FP64 argmin.
+    Explicit ``topk`` / ``rescore`` (arguments, then settings) override."""
+    precision = precision or getattr(s, "precision", "fp32")
     if precision not in ("fp32", "bf16", "fp8"):
         raise ValueError(f"precision must be fp32, bf16 or fp8, got {precision!r}")
-    K = topk or s.topk or (1 if precision == "fp32" else 8)
+    single = bool(getattr(s, "use_single_prec", False))
+    if rescore is None:
+        rescore = getattr(s, "rescore", None)
+    if rescore is None:
+        rescore = not (precision == "fp32" and single)
+    K = topk or getattr(s, "topk", 0) or (8 if precision != "fp32" else (4 if rescore else 1))
     if K not in (1, 2, 4, 8):
         raise ValueError("topk must be 1, 2, 4 or 8")
-    if rescore is None:
-        rescore = s.rescore
-    if rescore is None:
-        rescore = precision != "fp32"
     return precision, K, bool(rescore)
 
 
@@ -69,7 +78,7 @@ def slicing_and_match(ac, slices_, pixel_position_reciprocal, pixel_distance_rec
                              f"N_orientations = {N_orientations}")
     ref_orientations = np.asarray(ref_orientations)
     assert N_orientations % N_batch_size == 0, "N_orientations must be divisible by N_batch_size"
-    precision, K, do_rescore = _resolve_form(s, precision, topk, rescore)
+    precision, K, do_rescore = resolve_form(s, precision, topk, rescore)
 
     ac = np.asarray(ac)
     assert np.all(np.isreal(ac))

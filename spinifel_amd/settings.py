@@ -10,6 +10,12 @@ reads ``N_orientations``, ``N_batch_size``, ``reduced_det_shape``
 and the data prep ``N_clipping``, ``N_binning`` (spinifel/prep.py:116-134).
 The rest of the reference's TOML/CLI system is out of scope.
 
+``use_single_prec`` (spinifel/settings.py:303-307, default False) selects the
+distance precision as in the reference (f_type float64 unless set,
+spinifel/sequential/orientation_matching.py:32-37): with it False the FP32
+form rescores its best candidates in FP64, so the argmin is the FP64 one;
+with it True the FP32 form is the plain FP32 argmin.
+
 Fields added for this implementation: ``nufft_eps`` (the reference hard-codes
 1e-12 for finufft, spinifel/autocorrelation.py:59; FP32 slices cannot carry
 more than ~1e-7 so the default here is 1e-6), ``precision`` of the cross-term
@@ -33,9 +39,10 @@ class SpinifelSettings:
     N_phase_loops: int = 5          # ER/HIO/ER/shrink-wrap loops (settings.py:408-411)
     N_clipping: int = 1             # data prep clipping factor 2^n (settings.py:413-416)
     N_binning: int = 4              # data prep binning factor 2^n (settings.py:418-421)
+    use_single_prec: bool = False   # FP32 distances only (settings.py:303-307)
     precision: str = "fp32"     # 'fp32' | 'bf16' | 'fp8'
-    topk: int = 0               # 0: 1 for fp32, 8 otherwise
-    rescore: bool = None        # None: False for fp32, True otherwise
+    topk: int = 0               # 0: 8 for bf16 / fp8; fp32: 1 with use_single_prec, else 4
+    rescore: bool = None        # None: FP64 rescoring unless fp32 with use_single_prec
     device: int = 0
     verbose: bool = False
 

@@ -81,10 +81,13 @@ def test_oracle_reproduces_candidate_distances(gold):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
-def test_dropin_match_through_settings_equals_oracle(gold, precision):
+@pytest.mark.parametrize("form", ["fp32_plain", "fp32", "bf16", "fp8"])
+def test_dropin_match_through_settings_equals_oracle(gold, form):
+    # fp32_plain: use_single_prec, the plain FP32 argmin (K = 1, no rescoring);
+    # fp32: the default float64 semantics, FP32 candidates rescored in FP64
     from spinifel_amd import orientation_matching, settings as st
 
+    precision = form.split("_")[0]
     s = st.settings
     saved = dict(vars(s))
     try:
@@ -92,10 +95,11 @@ def test_dropin_match_through_settings_equals_oracle(gold, precision):
         s.N_batch_size = int(gold["N_batch_size"])
         s.reduced_det_shape = (4, 32, 128)
         s.oversampling = 1
+        s.use_single_prec = form == "fp32_plain"
         s.precision = precision
         s.topk = 0
         s.rescore = None
-        print(f"m81 golden: match() {precision}", flush=True)
+        print(f"m81 golden: match() {form}", flush=True)
         got = orientation_matching.match(gold["ac"], gold["images"], gold["ppr"], gold["pdr"],
                                          gold["ref_quats"])
         det = orientation_matching.slicing_and_match(gold["ac"], gold["images"], gold["ppr"],
@@ -109,7 +113,7 @@ def test_dropin_match_through_settings_equals_oracle(gold, precision):
     d = gold["top_d2"]
     margin = (d[:, 1] - d[:, 0]) / d[:, 0]
     bad = np.flatnonzero(~same)
-    print(f"m81 golden {precision}: {same.sum()} / {same.size} identical; "
+    print(f"m81 golden {form}: {same.sum()} / {same.size} identical; "
           f"flipped images {bad.tolist()} with FP64 margins {margin[bad].tolist()}", flush=True)
     assert same.all()
     np.testing.assert_array_equal(det["index"], gold["index"])

@@ -18,13 +18,16 @@ def _settings(W, n_batch):
     return s
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
-def test_snm_matches_oracle_class_path(precision):
+@pytest.mark.parametrize("precision,single", [("fp32", True), ("fp32", False), ("bf16", False)])
+def test_snm_matches_oracle_class_path(precision, single):
+    # single: use_single_prec (plain FP32 argmin); otherwise the FP32 form's
+    # candidates are rescored in FP64, the reference's float64 default
     from spinifel_amd.classes import NUFFT, SNM
     W = workload("m41_1k_px")
-    print(f"case: class path M=41, {len(W.ref_quats)} orientations, batches of 500, {precision}",
-          flush=True)
+    print(f"case: class path M=41, {len(W.ref_quats)} orientations, batches of 500, {precision}, "
+          f"use_single_prec={single}", flush=True)
     s = _settings(W, 500)
+    s.use_single_prec = single
     nufft = NUFFT(s, W.ppr, W.pdr, ref_orientations=W.ref_quats)
     snm = SNM(s, W.images, W.ppr, W.pdr, nufft, precision=precision)
     got = snm.slicing_and_match(W.ac)
