@@ -99,3 +99,24 @@ def test_default_phase_recovers_autocorrelation(torch_dev):
     assert ac_phased.shape == (M,) * 3 and sup.dtype == bool and rho.dtype == np.float64
     c = np.corrcoef(ac_phased.ravel(), target.ravel())[0, 1]
     assert c > 0.95, c
+
+
+def test_recenter_known_offset(torch_dev):
+    """The GPU recenter rolls an object centred at the signed voxel offset
+    (5, -3, 7) by (-5, 3, -7), density and support alike, exactly as the
+    oracle (pinned by its own known-answer test) does."""
+    import torch
+    from spinifel_amd import phasing
+    M = 21
+    rho = np.zeros((M,) * 3)
+    sup = np.zeros((M,) * 3, dtype=bool)
+    ax = [np.arange(4, 7), np.arange(-4, -1) % M, np.arange(6, 9)]
+    rho[np.ix_(*ax)] = np.random.default_rng(3).uniform(1.0, 1.5, (3, 3, 3))
+    sup[np.ix_(*ax)] = True
+    r_gpu, s_gpu = phasing.recenter(torch.as_tensor(rho, device=torch_dev),
+                                    torch.as_tensor(sup, device=torch_dev), M)
+    r_ora, s_ora = rho.copy(), sup.copy()
+    oph.recenter(r_ora, s_ora, M)
+    np.testing.assert_array_equal(r_gpu.cpu().numpy(), r_ora)
+    np.testing.assert_array_equal(s_gpu.cpu().numpy(), s_ora)
+    np.testing.assert_array_equal(r_ora, np.roll(rho, (-5, 3, -7), axis=(0, 1, 2)))

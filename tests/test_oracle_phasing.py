@@ -94,3 +94,43 @@ def test_phase_recovers_the_autocorrelation_of_a_compact_object():
     c = np.corrcoef(ac_phased.ravel(), target.ravel())[0, 1]
     assert c > 0.95, c
     assert rho.shape == (M,) * 3 and sup.dtype == bool
+
+
+def test_recenter_moves_a_known_offset_to_the_origin():
+    """Known answer for recenter / center_of_mass (sequential/phasing.py:46-81):
+    an object whose centre of mass sits at the signed voxel offset (5, -3, 7)
+    of the fftshifted grid is rolled by (-5, 3, -7), its support with it, so
+    the object ends up centred on voxel 0.  A flipped roll sign would move it
+    to (10, -6, 14); a transposed axis would leave it off centre."""
+    M = 21
+    rho = np.zeros((M, M, M))
+    sup = np.zeros((M, M, M), dtype=bool)
+    ax = [np.arange(4, 7), np.arange(-4, -1) % M, np.arange(6, 9)]     # centres 5, -3, 7
+    rho[np.ix_(*ax)] = np.random.default_rng(3).uniform(1.0, 1.0 + 1e-3, (3, 3, 3))
+    sup[np.ix_(*ax)] = True
+    rho0, sup0 = rho.copy(), sup.copy()
+    ls = np.linspace(-1, 1, M + 1)
+    ls = (ls[:-1] + ls[1:]) / 2
+    hkl = np.stack([np.fft.ifftshift(c) for c in np.meshgrid(ls, ls, ls, indexing="ij")])
+    np.testing.assert_array_equal(phasing.center_of_mass(rho, hkl, M), [5, -3, 7])
+    phasing.recenter(rho, sup, M)
+    want = np.roll(rho0, (-5, 3, -7), axis=(0, 1, 2))
+    np.testing.assert_array_equal(rho, want)
+    np.testing.assert_array_equal(sup, np.roll(sup0, (-5, 3, -7), axis=(0, 1, 2)))
+    centred = [np.arange(-1, 2) % M] * 3
+    assert rho[np.ix_(*centred)].min() >= 1.0
+    assert rho.sum() == pytest.approx(rho[np.ix_(*centred)].sum(), rel=1e-15)
+    np.testing.assert_array_equal(phasing.center_of_mass(rho, hkl, M), [0, 0, 0])
+
+
+def test_center_of_mass_uses_absolute_density_and_rounds_half_to_even():
+    """|rho| enters the centre of mass (phasing.py:55); a two-voxel object at
+    offsets 5 and 6 has its centre at 5.5, which np.round takes to 6."""
+    M = 15
+    ls = np.linspace(-1, 1, M + 1)
+    ls = (ls[:-1] + ls[1:]) / 2
+    hkl = np.stack([np.fft.ifftshift(c) for c in np.meshgrid(ls, ls, ls, indexing="ij")])
+    rho = np.zeros((M, M, M))
+    rho[5, 0, 0] = -2.0
+    rho[6, 0, 0] = 2.0
+    np.testing.assert_array_equal(phasing.center_of_mass(rho, hkl, M), [6, 0, 0])
