@@ -427,7 +427,8 @@ def run_batching(args, torch, engine, synth, log):
     h = hierarchy.Hierarchy(sizes=(1000, 10000, 100000), n_parents=4)
     t_h = time.perf_counter() - t0
     out = dict(n_batches=64, batch_sizes=sizes.tolist(), images=N_i, levels=h.sizes,
-               n_parents=4, beam=4, K=8, hierarchy_build_s=t_h, models={})
+               n_parents=4, level0_parts=2, level0_top=8, kept_per_entry=4, K=8,
+               hierarchy_build_s=t_h, models={})
     for M, atoms in MODELS.items():
         log(f"batching: model M={M}")
         ac = synth.make_model(M, atoms, args.seed)
@@ -445,7 +446,7 @@ def run_batching(args, torch, engine, synth, log):
         imgs = gpu_images(plan, true, q, scale, tiling, 1.0, args.seed + M, torch)
         iset = engine.ImageSet(imgs)
         del imgs
-        m = hierarchy.HierarchicalMatcher(hm, K=8, beam=4)
+        m = hierarchy.HierarchicalMatcher(hm, K=8)
         t = engine.Timers()
         m.run(iset, offs, timers=t)            # eager pass: per-kernel times
         torch.cuda.synchronize()

@@ -528,6 +528,15 @@ int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_i, int64_t 
                          int32_t *out, void *stream);
 
 /*
+ * In place, every entry of a row of idx (device int32 (rows, cols)) equal to
+ * an earlier entry of the same row becomes -1, so that a later
+ * mtip_group_by_winner leaves the repeat out (no counterpart in the
+ * reference: the batching form keeps each entry's best children as the
+ * next level's winners, and two entries can reach the same child).
+ */
+int mtip_unique_rows(int32_t *idx, int64_t rows, int cols, void *stream);
+
+/*
  * Per image, the K_out smallest distinct (value, index) candidates among
  * its n_in (in_idx / in_val, (N_i, n_in), index -1 = none): merges an
  * image's entries after mtip_group_topk.  Outputs (N_i, K_out) ascending,

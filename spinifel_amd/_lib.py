@@ -74,6 +74,7 @@ _SIGS = {
     "mtip_group_bad_count": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp]),
     "mtip_group_perm_offset": (_c.c_size_t, [_i64, _i64]),
     "mtip_merge_distinct": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp]),
+    "mtip_unique_rows": (_i32, [_vp, _i64, _i32, _vp]),
     "mtip_set_gemm_tile": (_i32, [_i32, _i32]),
     "mtip_gemm_tile": (_i32, [_i32, _i64, _i64]),
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,

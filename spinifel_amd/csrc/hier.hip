@@ -41,10 +41,14 @@ __global__ void k_count(const int32_t *__restrict__ win, int64_t win_stride, int
                         int *__restrict__ bad)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N_e) return;
-    const int32_t g = win[(e / n_win) * win_stride + e % n_win];
-    if (g >= 0 && g < n_groups) atomicAdd(&counts[g], 1);
-    else atomicAdd(bad, 1);
+    const int32_t g = e < N_e ? win[(e / n_win) * win_stride + e % n_win] : 0;
+    const bool ok = g >= 0 && g < n_groups;
+    if (e < N_e && ok) atomicAdd(&counts[g], 1);
+    // entries left out (often many: repeats marked -1) add to one counter,
+    // once per wave rather than once per entry
+    const uint64_t m = __ballot(e < N_e && !ok);
+    if (m && (threadIdx.x & 63) == (int)__ffsll((unsigned long long)m) - 1)
+        atomicAdd(bad, (int)__popcll(m));
 }
 
 // One workgroup: exclusive scans of the group sizes (image offsets) and of
@@ -99,10 +103,21 @@ __global__ void k_scatter(const int32_t *__restrict__ win, int64_t win_stride, i
                           int32_t *__restrict__ perm)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N_e) return;
-    const int32_t g = win[(e / n_win) * win_stride + e % n_win];
-    const int64_t slot = (g >= 0 && g < n_groups) ? g : n_groups;
-    perm[atomicAdd(&cursor[slot], 1)] = (int32_t)e;
+    const int32_t g = e < N_e ? win[(e / n_win) * win_stride + e % n_win] : 0;
+    const bool ok = g >= 0 && g < n_groups;
+    if (e < N_e && ok) perm[atomicAdd(&cursor[g], 1)] = (int32_t)e;
+    // the entries left out share one cursor: one atomic per wave, each lane
+    // placed by its rank among the wave's left-out entries
+    const uint64_t m = __ballot(e < N_e && !ok);
+    if (m) {
+        const int lane = threadIdx.x & 63, leader = (int)__ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[n_groups], (int)__popcll(m));
+        base = __shfl(base, leader, 64);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (e < N_e && !ok) perm[base + rank] = (int32_t)e;
+    }
 }
 
 // tasks[task_off[g] + t] = (g, first image position, end position)
@@ -417,6 +432,38 @@ extern "C" int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_
     GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
     MTIP_HIP_CHECK(hipMemcpyAsync(out, w.bad, sizeof(int32_t), hipMemcpyDeviceToDevice,
                                   (hipStream_t)stream));
+    return MTIP_OK;
+}
+
+namespace {
+
+// One wave per row: an entry equal to an earlier entry of its row becomes -1.
+__global__ __launch_bounds__(256) void k_unique_rows(int32_t *__restrict__ idx, int64_t rows,
+                                                     int cols)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    int32_t *r = idx + row * cols;
+    for (int c0 = 0; c0 < cols; c0 += 64) {
+        const int c = c0 + lane;
+        const int32_t v = c < cols ? r[c] : -1;
+        bool dup = false;
+        for (int j = 0; j < c && !dup; ++j) dup = r[j] == v;
+        if (c < cols && dup) r[c] = -1;
+    }
+}
+
+}  // namespace
+
+extern "C" int mtip_unique_rows(int32_t *idx, int64_t rows, int cols, void *stream)
+{
+    MTIP_REQUIRE(rows >= 0 && cols >= 1, MTIP_ERR_ARG, "mtip_unique_rows: bad sizes");
+    if (rows == 0) return MTIP_OK;
+    MTIP_REQUIRE(idx, MTIP_ERR_ARG, "mtip_unique_rows: null pointer");
+    hipLaunchKernelGGL(k_unique_rows, dim3((unsigned)mtip::ceil_div(rows, 4)), dim3(256), 0,
+                       (hipStream_t)stream, idx, rows, cols);
+    MTIP_HIP_CHECK(hipGetLastError());
     return MTIP_OK;
 }
 

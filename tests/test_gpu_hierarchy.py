@@ -174,7 +174,7 @@ def test_hierarchy_finds_planted_and_agrees_with_flat(small_hier):
     iset = engine.ImageSet(imgs)
     # the exact planting scale, so only the search is under test
     alpha = torch.as_tensor(scales, dtype=torch.float64).to(imgs.device)
-    hmatch = hierarchy.HierarchicalMatcher(hm, K=8, beam=4)
+    hmatch = hierarchy.HierarchicalMatcher(hm, K=8)
     r = hmatch.run(iset, offs, alpha=alpha)
     got = r["index"].cpu().numpy()
     flat = engine.match_resident(hm.levels[-1], iset, "bf16", K=8, alpha=alpha)
@@ -196,7 +196,7 @@ def test_hierarchy_noisy_images_agree_with_flat(small_hier):
     imgs = synth.poisson_images(sl, 5.0, rng)
     iset = engine.ImageSet(imgs.reshape(len(imgs), -1))
     offs = [0, 100, 300]
-    r = hierarchy.HierarchicalMatcher(hm, K=8, beam=4).run(iset, offs)
+    r = hierarchy.HierarchicalMatcher(hm, K=8).run(iset, offs)
     flat = engine.match_resident(hm.levels[-1], iset, "bf16", K=8, alpha=r["alpha"])
     agree = (r["index"].cpu().numpy() == flat["index"].cpu().numpy()).mean()
     assert agree >= 0.85
@@ -209,7 +209,7 @@ def test_graph_replay_equals_eager(small_hier):
     imgs = imgs + torch.rand_like(imgs)
     iset = engine.ImageSet(imgs)
     offs = [0, 17, 100, 257]
-    m = hierarchy.HierarchicalMatcher(hm, K=8, beam=4)
+    m = hierarchy.HierarchicalMatcher(hm, K=8)
     eager = m.run(iset, offs)
     e_idx, e_d = eager["index"].cpu().numpy(), eager["best_d64"].cpu().numpy()
     m.capture(iset, offs)
@@ -219,5 +219,50 @@ def test_graph_replay_equals_eager(small_hier):
         np.testing.assert_array_equal(out["index"].cpu().numpy(), e_idx)
         np.testing.assert_array_equal(out["best_d64"].cpu().numpy(), e_d)
         # unrelated eager work between replays
+        again = m.run(iset, offs)
+        np.testing.assert_array_equal(again["index"].cpu().numpy(), e_idx)
+
+
+@pytest.mark.parametrize("M,atoms", [(81, 800), (121, 2700)])
+def test_config3_hierarchy_agrees_with_flat_and_replays(M, atoms):
+    """The batching form at its configured size: 4 x 32 x 128 px, the 1k /
+    10k / 100k hierarchy, uneven Poisson batches (sizes log-uniform in
+    [16, 512], 2,000+ images in all); the hierarchical result equals the flat
+    BF16 + FP64 search over the 100k level for at least 99 % of the images, and
+    the captured graph's replays equal the eager pass bitwise."""
+    ppr, pdr = synth.detector(4, 32, 128)
+    E = float(pdr.max())
+    print(f"case: config-3 hierarchy M={M}", flush=True)
+    h = hierarchy.Hierarchy(sizes=(1000, 10000, 100000), n_parents=4)
+    ac = synth.make_model(M, atoms, 0)
+    plan = engine.NufftPlan(M, 1e-6, 0)
+    plan.set_model(ac)
+    q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).cuda()
+    tiling = engine.auto_tiling(plan, ppr.shape[1:])
+    hm = hierarchy.HierarchyModel(h, plan, q, np.pi / E, order=tiling)
+    sizes = hierarchy.log_uniform_batches(12, 16, 512, synth.rng_for(M, 7))
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N_i = int(offs[-1])
+    true = synth.random_quaternions(N_i, synth.rng_for(M, 8))
+    sl = plan.slices(torch.as_tensor(true).cuda(), q, np.pi / E, order=tiling)["s32"][:, :q.shape[1]]
+    sl.clamp_(min=0.0)
+    sl.mul_(1.0 / float(sl.mean()))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(M)
+    iset = engine.ImageSet(torch.poisson(sl, generator=g))
+    m = hierarchy.HierarchicalMatcher(hm, K=8)
+    eager = m.run(iset, offs)
+    e_idx, e_d = eager["index"].cpu().numpy(), eager["best_d64"].cpu().numpy()
+    flat = engine.match_resident(hm.levels[-1], iset, "bf16", K=8, alpha=eager["alpha"])
+    agree = float((e_idx == flat["index"].cpu().numpy()).mean())
+    print(f"config-3 M={M}: {N_i} images, agreement with the flat 100k search {agree:.4f}",
+          flush=True)
+    assert agree >= 0.99
+    m.capture(iset, offs)
+    for rep in range(3):
+        out = m.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out["index"].cpu().numpy(), e_idx)
+        np.testing.assert_array_equal(out["best_d64"].cpu().numpy(), e_d)
         again = m.run(iset, offs)
         np.testing.assert_array_equal(again["index"].cpu().numpy(), e_idx)
