@@ -486,8 +486,10 @@ def run_batching(args, torch, engine, synth, log):
 
 def run_serving(args, torch, engine, synth, log):
     """Four resident models (M = 81, 81, 121, 121 with 10k / 100k orientations),
-    Poisson bursts at 120/s of U[1, 512] images each, admitted by free HBM;
-    then the same bursts all at once (capacity)."""
+    admitted by free HBM: Poisson bursts of U[1, 512] images at 120 bursts/s
+    (light load) and at 1200 bursts/s (about 300k images/s offered), then the
+    light-load bursts all at once (capacity), as numpy requests and as pinned
+    host tensors."""
     from spinifel_amd import autotune, serving
     from spinifel_amd.quaternions import random_quaternions
     dev = torch.device("cuda", 0)
@@ -495,7 +497,8 @@ def run_serving(args, torch, engine, synth, log):
     E = float(pdr.max())
     q = torch.as_tensor(np.ascontiguousarray(ppr.reshape(3, -1))).to(dev)
     scale = np.pi / E
-    srv = serving.Server(precision="bf16", K=8, n_slots=4, max_batch_images=4096)
+    srv = serving.Server(precision="bf16", K=8, n_slots=4, max_batch_images=4096,
+                         compute_streams=2)
     pools = {}
     spec = [("m81_10k", 81, 10000), ("m81_100k", 81, 100000), ("m121_10k", 121, 10000),
             ("m121_100k", 121, 100000)]
@@ -515,31 +518,42 @@ def run_serving(args, torch, engine, synth, log):
         del imgs
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
-    rng = synth.rng_for(args.seed, 11)
+    pinned = {n: torch.from_numpy(p).pin_memory() for n, p in pools.items()}
     names = [s[0] for s in spec]
-    sched = serving.poisson_bursts(120.0, args.serve_seconds, 512, names, rng)
 
-    # request images drawn up front (host), so submission costs no sampling
-    reqs = [pools[m][rng.integers(0, len(pools[m]), n)] for (_, m, n) in sched]
-    it = iter(range(len(reqs)))
+    def requests(sched, seed, pool):
+        # each burst is a contiguous run of one model's images from a random
+        # start (a view: no host copy before submission)
+        rng = synth.rng_for(args.seed, seed)
+        starts = [int(rng.integers(0, 1024 - n + 1)) for (_, _, n) in sched]
+        reqs = [pool[m][a:a + n] for (_, m, n), a in zip(sched, starts)]
+        it = iter(reqs)
+        return lambda name, n: next(it)
 
-    def source(name, n):
-        return reqs[next(it)]
-
-    # warm-up: one request per model
-    for n in names:
+    for n in names:                        # warm-up: one request per model
         srv.match_now(n, pools[n][:8])
-    st = serving.run_load(srv, sched, source)
-    srv.completed = []
-    # capacity: the same requests submitted at once
-    it = iter(range(len(reqs)))
-    sat = serving.run_load(srv, [(0.0, m, n) for (_, m, n) in sched], source)
+    out = dict(models=[dict(name=n, M=M, orientations=r) for n, M, r in spec], setup_s=t_setup,
+               server=dict(n_slots=srv.n_slots, compute_streams=len(srv._streams),
+                           max_batch_images=srv.max_batch_images))
+    light = serving.poisson_bursts(120.0, args.serve_seconds, 512, names,
+                                   synth.rng_for(args.seed, 11))
+    loaded = serving.poisson_bursts(1200.0, 2.0, 512, names, synth.rng_for(args.seed, 12))
+    legs = [("poisson", light, pools, "Poisson 120 bursts/s, sizes U[1, 512], numpy"),
+            ("poisson_loaded", loaded, pools, "Poisson 1200 bursts/s, sizes U[1, 512], numpy"),
+            ("saturated", [(0.0, m, n) for (_, m, n) in light], pools,
+             "the light-load bursts submitted at once, numpy"),
+            ("saturated_pinned", [(0.0, m, n) for (_, m, n) in light], pinned,
+             "the light-load bursts submitted at once, pinned host tensors")]
+    for k, (leg, sched, pool, what) in enumerate(legs):
+        srv.reset_stats()
+        st = serving.run_load(srv, sched, requests(sched, 20 + k, pool))
+        dur = max(dt for dt, _, _ in sched) if sched else 0.0
+        st["offered_images_per_s"] = (sum(n for _, _, n in sched) / dur) if dur > 0 else None
+        st["arrival"] = what
+        out[leg] = st
+        log("serving", leg, json.dumps(st))
     free, total = serving.free_hbm(0)
-    out = dict(models=[dict(name=n, M=M, orientations=r) for n, M, r in spec],
-               setup_s=t_setup, arrival="Poisson 120 bursts/s, sizes U[1, 512]",
-               seconds=args.serve_seconds, poisson=st, saturated=sat,
-               hbm_free_gb=free / 1e9, hbm_total_gb=total / 1e9)
-    log("serving", json.dumps({"poisson": st, "saturated": sat}))
+    out.update(seconds=args.serve_seconds, hbm_free_gb=free / 1e9, hbm_total_gb=total / 1e9)
     return out
 
 

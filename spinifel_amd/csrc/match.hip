@@ -751,7 +751,11 @@ __global__ __launch_bounds__(256) void k_segment_ratio(const double *__restrict_
     const int64_t g = blockIdx.x;
     const int64_t a0 = off[g], a1 = off[g + 1];
     double a = 0, b = 0, c = 0, d = 0;
+    // unrolled so that the loads of several iterations are in flight at
+    // once (the sums keep their order)
+#pragma unroll 8
     for (int64_t k = a0 + threadIdx.x; k < a1; k += 256) { a += is[k]; b += iq[k]; }
+#pragma unroll 8
     for (int64_t k = threadIdx.x; k < ns; k += 256) { c += ss[k]; d += sq[k]; }
     a = wave_sum_f64(a);
     b = wave_sum_f64(b);

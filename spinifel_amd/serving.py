@@ -3,16 +3,25 @@
 A ``Server`` keeps several models resident in HBM, each with its reference
 slices computed once (FP32 for rescoring plus the centred residual in the
 GEMM format, bf16 by default).  Requests (a batch of images tagged with a
-model) are queued; a scheduler thread admits them onto ``n_slots`` HIP
-streams.  Whenever a slot is free it takes the oldest pending request and
-every other pending request for the same model (up to ``max_batch_images``),
-provided the device has enough free HBM for the batch (``mtip_mem_info``
-plus the blocks PyTorch's caching allocator holds free); requests that
-arrive while others run are therefore batched as soon as a slot frees up,
-and their work overlaps with the other slots on the device.  A batch that
-cannot be admitted while every slot is idle, even after the allocator's
-cache is released, fails its requests with MemoryError after
-``admit_timeout_s`` instead of waiting forever.
+model) are queued; a scheduler thread keeps up to ``n_slots`` batches in
+flight on ``compute_streams`` HIP streams.  Whenever a slot is free it takes
+the oldest pending request and every other pending request for the same
+model (up to ``max_batch_images``), provided the device has enough free HBM
+for the batch (``mtip_mem_info`` plus the blocks PyTorch's caching allocator
+holds free); requests that arrive while others run are therefore batched as
+soon as a slot frees up.  A batch that cannot be admitted while every slot
+is idle, even after the allocator's cache is released, fails its requests
+with MemoryError after ``admit_timeout_s`` instead of waiting forever.
+
+Image uploads run on one upload stream in admission order, ahead of the
+compute streams that wait for them, so the host link stays busy while
+admitted batches compute.  Requests handed in as pinned host tensors are
+uploaded from where they are, numpy requests through a pinned staging buffer
+per slot.  The upload stream never waits on a compute stream, and the
+compute streams are few (two by default): HIP maps streams onto a handful of
+hardware queues (four unless GPU_MAX_HW_QUEUES says otherwise), and a stream
+that shares a queue with a long GEMM has its copies and markers held behind
+that kernel.
 
 Each request is scaled by its own std ratio (``mtip_segment_scaling_ratio``)
 and every per-image quantity (cross term, top-K keys, FP64 rescore) depends
@@ -49,9 +58,10 @@ class ServedModel:
 class Request:
     rid: int
     model: str
-    images: np.ndarray
+    images: object                # numpy (N, N_p) float32 or a pinned host tensor
     t_submit: float
     future: Future = field(default_factory=Future)
+    n_images: int = 0
     t_start: float = 0.0
     t_done: float = 0.0
     batch_size: int = 0
@@ -75,7 +85,8 @@ def free_hbm(device=0):
 
 class Server:
     def __init__(self, precision="bf16", K=8, n_slots=4, max_batch_images=4096,
-                 mem_reserve_bytes=8 << 30, device=0, eps=1e-6, admit_timeout_s=30.0):
+                 mem_reserve_bytes=8 << 30, device=0, eps=1e-6, admit_timeout_s=30.0,
+                 stats_window=100000, compute_streams=2):
         engine.require_gpu()
         self.precision = precision
         self.K = K
@@ -90,7 +101,10 @@ class Server:
         self._plans = {}
         self._queue = deque()
         self._lock = threading.Condition()
-        self._streams = [torch.cuda.Stream(device=self.device) for _ in range(n_slots)]
+        # batches in flight (slots) share `compute_streams` compute streams
+        self._streams = [torch.cuda.Stream(device=self.device)
+                         for _ in range(max(1, min(compute_streams, n_slots)))]
+        self._upload_stream = torch.cuda.Stream(device=self.device)
         self._busy = [None] * n_slots          # (requests, event, outputs) per slot
         # per slot a pinned host staging buffer, reused while the slot is free:
         # requests are copied into it once and uploaded from it asynchronously
@@ -100,7 +114,9 @@ class Server:
         self.error = None
         self._stop = False
         self._thread = None
-        self.completed = []
+        # the last stats_window finished requests (timings only: a request
+        # drops its images once its result is set)
+        self.completed = deque(maxlen=stats_window)
 
     # ------------------------------------------------------------- models
     def add_model(self, name, ac, ref_quats, pixel_position_reciprocal, pixel_distance_reciprocal,
@@ -134,14 +150,24 @@ class Server:
 
     # ------------------------------------------------------------- requests
     def submit(self, model, images):
+        """Queue a request; returns a Future of its Result.  images: numpy
+        (copied into a pinned staging buffer at launch) or a pinned host
+        torch float32 tensor (uploaded from where it is, no host copy; the
+        caller must not change it before the future completes)."""
         if model not in self.models:
             raise KeyError(model)
-        images = np.ascontiguousarray(images, np.float32).reshape(len(images), -1)
+        if torch.is_tensor(images):
+            if images.device.type != "cpu" or not images.is_pinned() or \
+                    images.dtype != torch.float32:
+                raise ValueError("tensor requests must be pinned host float32 tensors")
+            images = images.reshape(images.shape[0], -1).contiguous()
+        else:
+            images = np.ascontiguousarray(images, np.float32).reshape(len(images), -1)
         if images.shape[1] != self.models[model].N_p:
             raise ValueError(f"images have {images.shape[1]} pixels, model {model} "
                              f"{self.models[model].N_p}")
         with self._lock:
-            r = Request(self._next_id, model, images, time.perf_counter())
+            r = Request(self._next_id, model, images, time.perf_counter(), n_images=len(images))
             self._next_id += 1
             self._queue.append(r)
             self._lock.notify()
@@ -149,8 +175,9 @@ class Server:
 
     def match_now(self, model, images):
         """Synchronous single-request match (no queue), the 'alone' reference."""
-        r = Request(-1, model, np.ascontiguousarray(images, np.float32).reshape(len(images), -1),
-                    time.perf_counter())
+        if not torch.is_tensor(images):
+            images = np.ascontiguousarray(images, np.float32).reshape(len(images), -1)
+        r = Request(-1, model, images, time.perf_counter(), n_images=len(images))
         with self._now_lock:
             out = self._launch([r], torch.cuda.current_stream(self.device), self.n_slots)
             torch.cuda.synchronize(self.device)
@@ -169,27 +196,51 @@ class Server:
             self._stage[slot][n_p] = buf
         return buf
 
-    def _staged(self, reqs, slot, n, n_p):
-        """The batch's images in the slot's pinned buffer (n, n_p) float32.  The
-        buffer is only rewritten once the slot's previous batch has retired,
-        i.e. after its upload completed."""
-        buf = self._stage_buffer(slot, n, n_p)
+    def _upload(self, reqs, slot, n, n_p, stream):
+        """The batch's images on the device (n, n_p) float32.
+
+        The copies go on the server's one upload stream, in admission order,
+        and `stream` (the slot's compute stream) waits for them: uploads of
+        admitted batches run back to back over the host link while earlier
+        batches compute.  Pinned tensor requests go up directly; numpy
+        requests are first copied into the slot's pinned buffer, which is
+        only rewritten once the slot's previous batch has retired (its
+        upload completed)."""
+        up = self._upload_stream
+        buf = None
         a = 0
-        for r in reqs:
-            k = len(r.images)
-            # torch's copy runs on its intra-op threads, outside the interpreter lock
-            buf[a:a + k].copy_(torch.from_numpy(r.images))
-            a += k
-        return buf[:n]
+        with torch.cuda.stream(up):
+            # from the upload stream's pool: a block there is freed only when
+            # the batch that read it has retired, so the copies can write it
+            # at once.  The upload stream never waits on a compute stream
+            # (a long kernel elsewhere cannot hold the copies back).
+            dev_buf = torch.empty((n, n_p), dtype=torch.float32, device=self.device)
+            for r in reqs:
+                k = r.n_images
+                if torch.is_tensor(r.images):
+                    dev_buf[a:a + k].copy_(r.images, non_blocking=True)
+                else:
+                    if buf is None:
+                        buf = self._stage_buffer(slot, n, n_p)
+                    # torch's copy runs on its intra-op threads, outside the interpreter lock
+                    buf[a:a + k].copy_(torch.from_numpy(r.images))
+                    dev_buf[a:a + k].copy_(buf[a:a + k], non_blocking=True)
+                a += k
+        stream.wait_stream(up)
+        return dev_buf
 
     def _launch(self, reqs, stream, slot=0):
         m = self.models[reqs[0].model]
-        sizes = [len(r.images) for r in reqs]
+        sizes = [r.n_images for r in reqs]
         offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        host = self._staged(reqs, slot, int(offs[-1]), m.N_p)
+        dev_imgs = self._upload(reqs, slot, int(offs[-1]), m.N_p, stream)
         with torch.cuda.stream(stream):
-            imgs = engine.ImageSet(host.to(self.device, non_blocking=True), self.device, stream)
-            alpha = engine.segment_scaling_ratio(imgs.row_sum, imgs.row_sumsq, offs,
+            # request offsets go up asynchronously from pinned memory: a
+            # pageable copy would block this thread until the slot's image
+            # upload ahead of it has finished
+            offs_dev = torch.from_numpy(offs).pin_memory().to(self.device, non_blocking=True)
+            imgs = engine.ImageSet(dev_imgs, self.device, stream)
+            alpha = engine.segment_scaling_ratio(imgs.row_sum, imgs.row_sumsq, offs_dev,
                                                  m.slices.row_sum, m.slices.row_sumsq, m.N_p, stream)
             D = imgs.operand(self.precision, alpha, m.slices.center, stream)
             one = engine._one(self.device)
@@ -200,7 +251,7 @@ class Server:
             bd = rs["best_d64"].to("cpu", non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        return dict(best=best, bd=bd, event=ev, offs=offs, keep=(imgs, D, idx, rs))
+        return dict(best=best, bd=bd, event=ev, offs=offs, keep=(offs_dev, imgs, D, idx, rs))
 
     def _collect(self, reqs, out):
         now = time.perf_counter()
@@ -236,10 +287,10 @@ class Server:
         for r in list(self._queue):
             if r.model != first.model:
                 continue
-            if batch and n + len(r.images) > self.max_batch_images:
+            if batch and n + r.n_images > self.max_batch_images:
                 break
             batch.append(r)
-            n += len(r.images)
+            n += r.n_images
         need = self._batch_bytes(n, m.N_p)
         if self._available_bytes() - self.mem_reserve < need:
             if all(b is None for b in self._busy):
@@ -302,6 +353,7 @@ class Server:
                     for r, res in zip(reqs, self._collect(reqs, out)):
                         r.t_done = time.perf_counter()
                         r.batch_size = res.batch_images
+                        r.images = None
                         self.completed.append(r)
                         r.future.set_result(res)
                     self._busy[s] = None
@@ -318,15 +370,19 @@ class Server:
                         t = time.perf_counter()
                         for r in batch:
                             r.t_start = t
-                        self._busy[s] = (batch, self._launch(batch, self._streams[s], s))
+                        stream = self._streams[s % len(self._streams)]
+                        self._busy[s] = (batch, self._launch(batch, stream, s))
                         progressed = True
                 if not progressed:
                     self._lock.wait(timeout=0.0005)
 
     # ------------------------------------------------------------- stats
+    def reset_stats(self):
+        self.completed.clear()
+
     def stats(self):
         lat = np.array([r.t_done - r.t_submit for r in self.completed])
-        n_img = sum(len(r.images) for r in self.completed)
+        n_img = sum(r.n_images for r in self.completed)
         if not len(lat):
             return {}
         t0 = min(r.t_submit for r in self.completed)

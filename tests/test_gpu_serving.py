@@ -56,6 +56,35 @@ def test_requests_alone_equal_in_mix(server):
     assert st["requests"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
 
 
+def test_pinned_tensor_requests_equal_numpy(server):
+    """Requests handed in as pinned host tensors (uploaded without a host copy)
+    batch with numpy requests and give bitwise the numpy results; a tensor that
+    is not pinned host float32 is refused."""
+    import torch
+    srv, models = server
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(16):
+        name = "ab"[int(rng.integers(0, 2))]
+        sel = rng.choice(400, int(rng.integers(1, 120)), replace=False)
+        reqs.append((name, models[name]["imgs"][sel].astype(np.float32), k % 2 == 0))
+    alone = [srv.match_now(name, imgs) for name, imgs, _ in reqs]
+    srv.start()
+    try:
+        futs = [srv.submit(name, torch.from_numpy(imgs.reshape(len(imgs), -1)).pin_memory()
+                           if pinned else imgs)
+                for name, imgs, pinned in reqs]
+        mixed = [f.result(timeout=120) for f in futs]
+        with pytest.raises(ValueError):
+            srv.submit("a", torch.from_numpy(reqs[0][1].reshape(len(reqs[0][1]), -1)))
+    finally:
+        srv.stop()
+    assert max(r.batch_images for r in mixed) > max(len(i) for _, i, _ in reqs), "no batching"
+    for a, m in zip(alone, mixed):
+        np.testing.assert_array_equal(a.index, m.index)
+        np.testing.assert_array_equal(a.dist64, m.dist64)
+
+
 def test_served_result_matches_oracle(server):
     srv, models = server
     m = models["b"]
