@@ -132,16 +132,17 @@ def run_form(form, W, args, torch, engine, timers):
         sl = engine.SliceSet.compute(plan, W["quats"], W["q"], W["scale"], timers=t, order=W["order"])
         return sl, engine.match_resident(sl, imgs, precision, K=K, rescore_fp64=resc, timers=t)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    timers.clear()
-    lat = []
-    # the interpreter's cyclic collector runs between steps, not inside them
-    # (a collection inside a step stalls the launches and the GPU idles)
+    # the interpreter's cyclic collector runs before the warm-up, not inside
+    # the steps (a collection inside a step stalls the launches and the GPU
+    # idles) nor between warm-up and timing (the GPU clock drops while idle)
     gc.collect()
     gc.disable()
     try:
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        timers.clear()
+        lat = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             s0 = time.perf_counter()

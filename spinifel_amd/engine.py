@@ -566,6 +566,63 @@ def col_mean(x, n_cols=None, stream=None, row_step=1):
 _CENTER_ROWS = 1024
 
 
+# pinned host staging and a copy stream per device for upload_rows
+_STAGING = {}
+_UPLOAD_STREAMS = {}
+
+
+def upload_rows(rows, device, chunk_rows=1024):
+    """Host rows (N, P) -> device float32 tensor (N, P), uploaded on a copy
+    stream of its own while the current stream keeps running.
+
+    The rows go through one pinned staging buffer kept per device (grown to
+    the largest upload and reused; ``release_staging`` frees it), chunk by
+    chunk: the host copy of a chunk overlaps the DMA of the one before, and
+    no chunk waits on the current stream, so an upload issued after a kernel
+    runs beside it.  The current stream waits for the whole upload before
+    anything it issues later.  The device tensor comes from the copy
+    stream's pool and is marked in use by the current stream.
+    """
+    dev = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+    arr = np.ascontiguousarray(rows, dtype=np.float32)
+    arr = arr.reshape(arr.shape[0], -1)
+    n, p = arr.shape
+    cur = torch.cuda.current_stream(dev)
+    up = _UPLOAD_STREAMS.get(dev)
+    if up is None:
+        up = _UPLOAD_STREAMS[dev] = torch.cuda.Stream(device=dev)
+    st = _STAGING.get(dev)
+    if st is None or st["buf"].numel() < arr.size:
+        if st is not None:
+            st["done"].synchronize()
+        st = _STAGING[dev] = dict(buf=torch.empty(max(arr.size, 1), dtype=torch.float32,
+                                                  pin_memory=True),
+                                  done=torch.cuda.Event())
+    else:
+        st["done"].synchronize()        # the previous upload from the buffer has finished
+    with torch.cuda.stream(up):
+        out = torch.empty((n, p), dtype=torch.float32, device=dev)
+    out.record_stream(cur)
+    host = st["buf"][:arr.size].view(n, p)
+    src = torch.from_numpy(arr)
+    step = max(1, int(chunk_rows))
+    with torch.cuda.stream(up):
+        for a in range(0, n, step):
+            b = min(n, a + step)
+            host[a:b].copy_(src[a:b])   # torch's threaded copy, outside the interpreter lock
+            out[a:b].copy_(host[a:b], non_blocking=True)
+        st["done"].record(up)
+    cur.wait_event(st["done"])
+    return out
+
+
+def release_staging():
+    """Free upload_rows' pinned staging buffers (after their uploads finish)."""
+    for st in _STAGING.values():
+        st["done"].synchronize()
+    _STAGING.clear()
+
+
 class ImageSet:
     """Images resident in HBM: FP32 rows padded to a multiple of 16 pixels,
     FP64 per-row sums, and lazily built bf16 / e4m3 copies."""

@@ -94,7 +94,9 @@ def slicing_and_match(ac, slices_, pixel_position_reciprocal, pixel_distance_rec
     st_slice = time.monotonic()
     sl = engine.SliceSet.compute(plan, ref_orientations[:N_orientations], ppr.reshape(3, -1),
                                  scale, precision, order=engine.auto_tiling(plan, ppr.shape[1:]))
-    imgs = engine.ImageSet(slices_.reshape(N_slices, N_pixels), plan.device)
+    # the images go up on a copy stream while the slicing runs
+    imgs = engine.ImageSet(engine.upload_rows(slices_.reshape(N_slices, N_pixels), plan.device),
+                           plan.device)
     if int(sl.bad.item()) > 0:
         raise AssertionError(f"{int(sl.bad.item())} NU points outside [-3pi, 3pi)")
     en_slice = time.monotonic()
