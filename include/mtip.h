@@ -453,7 +453,8 @@ int mtip_rescore(const float *slices, int64_t ld_s, int64_t N_o, const float *im
  * Replaces cudaEuclideanDistance(refImgs, dataImgs, numRefImgs,
  * numDataImgs, totalPixels, deviceId) (orientation_matching_sp.cu:612-679)
  * minus its host copies and its dropping of the last totalPixels % 16
- * pixels (:110).  totalPixels must be a multiple of 32.
+ * pixels (:110).  totalPixels must be a multiple of 4 (rows are read in
+ * 16-byte chunks); ref and data 16-byte aligned.
  */
 int mtip_euclidean_distance(const float *ref, const float *data, int64_t numRef, int64_t numData,
                             int64_t totalPixels, float *euDist, void *stream);

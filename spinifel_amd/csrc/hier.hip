@@ -149,10 +149,15 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 // path, so v ranks like the squared distance).  kd: pixels summed, a
 // multiple of 64 within both row pitches; columns past N_p are zero.
 //
-// Wave w takes children c0 + 32 w .. c0 + 32 w + 31 of each pass of 128
-// and the full pixel range, for two 32-entry column blocks, so a child's
-// row is read once per task; the waves' running top-K lists are merged in
-// LDS at the end.
+// Wave w takes children c0 + 32 w .. c0 + 32 w + 31 of each pass of 128,
+// for two 32-entry column blocks, reading its children's rows straight from
+// memory (each child row once per task).  The 64 entries' rows are shared
+// by the four waves: each 64-pixel chunk of them is staged once in LDS
+// (double-buffered, the next chunk loaded during this one's MFMAs), so an
+// entry row is read from memory once per pass instead of once per wave.
+// The waves' running top-K lists are merged in LDS at the end.
+constexpr int kEntStride = 72;       // bf16 per staged entry row: 64 + 8 (144 bytes)
+
 template <int K>
 __global__ __launch_bounds__(256) void k_group_topk(
     const int4 *__restrict__ tasks, const int32_t *__restrict__ n_tasks,
@@ -162,20 +167,23 @@ __global__ __launch_bounds__(256) void k_group_topk(
     int n_win, int32_t *__restrict__ out_idx, float *__restrict__ out_val)
 {
     __shared__ uint64_t s_top[4][kTaskImages][K];
+    __shared__ __attribute__((aligned(16))) uint16_t s_ent[2][kTaskImages * kEntStride];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
     const int nt = n_tasks[0];
+    const int64_t nk = kd / 64;
+    // staging: entry rows tid / 8 and tid / 8 + 32, 16-byte piece tid % 8 of a chunk
+    const int st_row = tid >> 3, st_col = (tid & 7) * 8;
 
     for (int task = blockIdx.x; task < nt; task += gridDim.x) {
         const int4 tk = tasks[task];
         const int g = tk.x, p0 = tk.y, p1 = tk.z;
         const int c_begin = child_off[g], c_end = child_off[g + 1];
-        // this lane's entry rows (B operand column l32 of blocks 0 and 1)
-        const uint16_t *drow[2];
+        const uint16_t *erow[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int pe = p0 + 32 * j + l32;
-            drow[j] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
+            const int pe = p0 + st_row + 32 * j;
+            erow[j] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
         }
         uint64_t top[2][K];
 #pragma unroll
@@ -183,7 +191,8 @@ __global__ __launch_bounds__(256) void k_group_topk(
 #pragma unroll
             for (int r = 0; r < K; ++r) top[j][r] = KEY_NONE;
 
-        for (int c0 = c_begin + 32 * wave; c0 < c_end; c0 += 128) {
+        for (int cp = c_begin; cp < c_end; cp += 128) {
+            const int c0 = cp + 32 * wave;
             const int crow = c0 + l32;
             const int child = crow < c_end ? child_idx[crow] : -1;
             const uint16_t *srow = child >= 0 ? S + (int64_t)child * lds : nullptr;
@@ -192,38 +201,50 @@ __global__ __launch_bounds__(256) void k_group_topk(
             for (int j = 0; j < 2; ++j)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
-            // 64-element chunks, the next chunk's loads in flight during this one's MFMAs
-            uint4 a[4], b0[4], b1[4];
+            // chunk 0: entries into LDS buffer 0, this wave's children into registers
+            uint4 a[4], en[2];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int64_t e = half * 32 + 8 * s;
-                a[s] = ld16(srow, e);
-                b0[s] = ld16(drow[0], e);
-                b1[s] = ld16(drow[1], e);
-            }
-            for (int64_t k = 0; k < kd; k += 64) {
-                uint4 an[4], bn0[4], bn1[4];
-                const bool more = k + 64 < kd;
+            for (int j = 0; j < 2; ++j) en[j] = ld16(erow[j], st_col);
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int64_t e = k + 64 + half * 32 + 8 * s;
-                    an[s] = more ? ld16(srow, e) : make_uint4(0, 0, 0, 0);
-                    bn0[s] = more ? ld16(drow[0], e) : make_uint4(0, 0, 0, 0);
-                    bn1[s] = more ? ld16(drow[1], e) : make_uint4(0, 0, 0, 0);
+            for (int j = 0; j < 2; ++j)
+                *(uint4 *)(s_ent[0] + (st_row + 32 * j) * kEntStride + st_col) = en[j];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[s] = ld16(srow, half * 32 + 8 * s);
+            __syncthreads();
+            for (int64_t kc = 0; kc < nk; ++kc) {
+                const bool more = kc + 1 < nk;
+                const int64_t e1 = (kc + 1) * 64;
+                uint4 an[4];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    en[j] = more ? ld16(erow[j], e1 + st_col) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    an[s] = more ? ld16(srow, e1 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                const uint16_t *eb = s_ent[kc & 1];
+                if (c0 < c_end) {       // wave-uniform: a wave past the children only stages
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int off = half * 32 + 8 * s;
+                        const uint4 b0 = *(const uint4 *)(eb + l32 * kEntStride + off);
+                        const uint4 b1 = *(const uint4 *)(eb + (32 + l32) * kEntStride + off);
+                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            as_bf16x8(a[s]), as_bf16x8(b0), acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            as_bf16x8(a[s]), as_bf16x8(b1), acc[1], 0, 0, 0);
+                    }
+                }
+                // the other buffer was last read in chunk kc - 1, before the
+                // barrier that ended it
+                if (more) {
+                    uint16_t *nb = s_ent[(kc + 1) & 1];
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        *(uint4 *)(nb + (st_row + 32 * j) * kEntStride + st_col) = en[j];
                 }
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b0[s]),
-                                                                     acc[0], 0, 0, 0);
-                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a[s]), as_bf16x8(b1[s]),
-                                                                     acc[1], 0, 0, 0);
-                }
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    a[s] = an[s];
-                    b0[s] = bn0[s];
-                    b1[s] = bn1[s];
-                }
+                for (int s = 0; s < 4; ++s) a[s] = an[s];
+                __syncthreads();
             }
             // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
 #pragma unroll

@@ -1446,8 +1446,11 @@ extern "C" int mtip_euclidean_distance(const float *ref, const float *data, int6
     MTIP_REQUIRE(numRef >= 0 && numData >= 0 && totalPixels >= 0, MTIP_ERR_ARG,
                  "mtip_euclidean_distance: negative size");
     if (numRef == 0 || numData == 0) return MTIP_OK;
-    MTIP_REQUIRE(totalPixels > 0 && totalPixels % 32 == 0, MTIP_ERR_ARG,
-                 "mtip_euclidean_distance: totalPixels must be a positive multiple of 32");
+    // rows are staged in 16-byte chunks that lie wholly inside or wholly
+    // past a row (the part past it reads as zero), so a row must be a whole
+    // number of chunks
+    MTIP_REQUIRE(totalPixels > 0 && totalPixels % 4 == 0, MTIP_ERR_ARG,
+                 "mtip_euclidean_distance: totalPixels must be a positive multiple of 4");
     MTIP_REQUIRE(ref && data && euDist, MTIP_ERR_ARG, "mtip_euclidean_distance: null pointer");
     MTIP_REQUIRE(((uintptr_t)ref % 16) == 0 && ((uintptr_t)data % 16) == 0, MTIP_ERR_ARG,
                  "mtip_euclidean_distance: inputs must be 16-byte aligned");

@@ -49,6 +49,8 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert rc == _lib.MTIP_ERR_ARG and "FP8" in _lib.last_error()
     rc = lib.mtip_heap_sort(nul, 4, 4, 9, nul, nul)
     assert rc == _lib.MTIP_ERR_ARG
+    rc = lib.mtip_euclidean_distance(fake, fake, 4, 4, 18, fake, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "multiple of 4" in _lib.last_error()
     rc = lib.mtip_plan_create(1, 1e-6, 0, ctypes.byref(ctypes.c_void_p()))
     assert rc == _lib.MTIP_ERR_ARG and "M must be" in _lib.last_error()
     rc = lib.mtip_residual_rows(nul, 4, 8, 4, nul, 0, nul, 0, nul, 8, nul, nul, nul)

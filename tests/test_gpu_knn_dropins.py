@@ -88,3 +88,19 @@ def test_heap_sort_any_total_nn(totalNN):
     if totalNN > n_ref:
         want = np.concatenate([want, np.full((37, totalNN - n_ref), -1)], axis=1)
     np.testing.assert_array_equal(got, want)
+
+
+def test_euclidean_distance_rows_not_a_multiple_of_32():
+    """mtip_euclidean_distance on rows of 36 and 100 pixels (multiples of 4,
+    not of 32) against numpy FP64: the part of the last 128-byte step past the
+    row reads as zero."""
+    import torch
+    from spinifel_amd import engine
+    rng = np.random.default_rng(3)
+    for n_pix in (36, 100):
+        ref = rng.random((300, n_pix)).astype(np.float32)
+        dat = rng.random((170, n_pix)).astype(np.float32)
+        got = engine.euclidean_distance(torch.from_numpy(ref).cuda(),
+                                        torch.from_numpy(dat).cuda()).cpu().numpy()
+        want = ((dat.astype(np.float64)[:, None, :] - ref.astype(np.float64)[None]) ** 2).sum(2)
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
