@@ -152,11 +152,17 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 // Wave w takes children c0 + 32 w .. c0 + 32 w + 31 of each pass of 128,
 // for two 32-entry column blocks, reading its children's rows straight from
 // memory (each child row once per task).  The 64 entries' rows are shared
-// by the four waves: each 64-pixel chunk of them is staged once in LDS
-// (double-buffered, the next chunk loaded during this one's MFMAs), so an
-// entry row is read from memory once per pass instead of once per wave.
-// The waves' running top-K lists are merged in LDS at the end.
-constexpr int kEntStride = 72;       // bf16 per staged entry row: 64 + 8 (144 bytes)
+// by the four waves: each chunk of kChunk pixels of them is staged once in
+// LDS (double-buffered), so an entry row is read from memory once per pass
+// instead of once per wave.  A chunk's loads (children and entries) are
+// issued a whole chunk ahead of their MFMAs: the kernel is bound by that
+// memory latency, so a chunk is two 64-pixel MFMA steps long.  The waves'
+// running top-K lists are merged in LDS at the end.
+constexpr int kXcdsH = 8;                      // XCDs of the device (workgroups dealt round robin)
+constexpr int kChunkSteps = 2;                 // 64-pixel MFMA steps per chunk
+constexpr int kChunk = 64 * kChunkSteps;       // pixels per chunk
+constexpr int kEntStride = kChunk + 8;         // bf16 per staged entry row (16-byte multiple)
+constexpr int kEntPieces = kTaskImages * kChunk / 8 / 256;   // 16-byte pieces per thread
 
 template <int K>
 __global__ __launch_bounds__(256) void k_group_topk(
@@ -171,19 +177,35 @@ __global__ __launch_bounds__(256) void k_group_topk(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
     const int nt = n_tasks[0];
-    const int64_t nk = kd / 64;
-    // staging: entry rows tid / 8 and tid / 8 + 32, 16-byte piece tid % 8 of a chunk
-    const int st_row = tid >> 3, st_col = (tid & 7) * 8;
+    // staging: piece q of thread tid is entry row (tid + 256 q) / (kChunk / 8),
+    // 8-element column ((tid + 256 q) % (kChunk / 8)) * 8 of a chunk
+    constexpr int kRowPieces = kChunk / 8;
+    int st_row[kEntPieces], st_col[kEntPieces];
+#pragma unroll
+    for (int q = 0; q < kEntPieces; ++q) {
+        st_row[q] = (tid + 256 * q) / kRowPieces;
+        st_col[q] = ((tid + 256 * q) % kRowPieces) * 8;
+    }
+    // kd is a multiple of 64; a last chunk of one step reads zeros past kd
+    const int64_t nk = (kd + kChunk - 1) / kChunk;
 
-    for (int task = blockIdx.x; task < nt; task += gridDim.x) {
+    // workgroup b runs on XCD b % 8: XCD x takes the x-th of 8 contiguous
+    // task ranges, dealt round robin to its workgroups, so the tasks of one
+    // group (adjacent in the list) run together on one XCD and read their
+    // children's rows through one L2
+    const int xcd = blockIdx.x % kXcdsH, local = blockIdx.x / kXcdsH;
+    const int xcd_blocks = ((int)gridDim.x - xcd + kXcdsH - 1) / kXcdsH;
+    const int t_end = (int)((int64_t)nt * (xcd + 1) / kXcdsH);
+    for (int task = (int)((int64_t)nt * xcd / kXcdsH) + local; task < t_end;
+         task += xcd_blocks) {
         const int4 tk = tasks[task];
         const int g = tk.x, p0 = tk.y, p1 = tk.z;
         const int c_begin = child_off[g], c_end = child_off[g + 1];
-        const uint16_t *erow[2];
+        const uint16_t *erow[kEntPieces];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int pe = p0 + st_row + 32 * j;
-            erow[j] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
+        for (int q = 0; q < kEntPieces; ++q) {
+            const int pe = p0 + st_row[q];
+            erow[q] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
         }
         uint64_t top[2][K];
 #pragma unroll
@@ -191,6 +213,10 @@ __global__ __launch_bounds__(256) void k_group_topk(
 #pragma unroll
             for (int r = 0; r < K; ++r) top[j][r] = KEY_NONE;
 
+        // element e of a row, or zero past kd (or for a missing row)
+        auto ldk = [&](const uint16_t *row, int64_t e) __attribute__((always_inline)) {
+            return e < kd ? ld16(row, e) : make_uint4(0, 0, 0, 0);
+        };
         for (int cp = c_begin; cp < c_end; cp += 128) {
             const int c0 = cp + 32 * wave;
             const int crow = c0 + l32;
@@ -202,30 +228,32 @@ __global__ __launch_bounds__(256) void k_group_topk(
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
             // chunk 0: entries into LDS buffer 0, this wave's children into registers
-            uint4 a[4], en[2];
+            uint4 a[4 * kChunkSteps], en[kEntPieces];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) en[j] = ld16(erow[j], st_col);
+            for (int q = 0; q < kEntPieces; ++q) en[q] = ldk(erow[q], st_col[q]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                *(uint4 *)(s_ent[0] + (st_row + 32 * j) * kEntStride + st_col) = en[j];
+            for (int q = 0; q < kEntPieces; ++q)
+                *(uint4 *)(s_ent[0] + st_row[q] * kEntStride + st_col[q]) = en[q];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) a[s] = ld16(srow, half * 32 + 8 * s);
+            for (int s = 0; s < 4 * kChunkSteps; ++s)
+                a[s] = ldk(srow, 64 * (s >> 2) + half * 32 + 8 * (s & 3));
             __syncthreads();
             for (int64_t kc = 0; kc < nk; ++kc) {
                 const bool more = kc + 1 < nk;
-                const int64_t e1 = (kc + 1) * 64;
-                uint4 an[4];
+                const int64_t e1 = (kc + 1) * kChunk;
+                uint4 an[4 * kChunkSteps];
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    en[j] = more ? ld16(erow[j], e1 + st_col) : make_uint4(0, 0, 0, 0);
+                for (int q = 0; q < kEntPieces; ++q)
+                    en[q] = more ? ldk(erow[q], e1 + st_col[q]) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    an[s] = more ? ld16(srow, e1 + half * 32 + 8 * s) : make_uint4(0, 0, 0, 0);
+                for (int s = 0; s < 4 * kChunkSteps; ++s)
+                    an[s] = more ? ldk(srow, e1 + 64 * (s >> 2) + half * 32 + 8 * (s & 3))
+                                 : make_uint4(0, 0, 0, 0);
                 const uint16_t *eb = s_ent[kc & 1];
                 if (c0 < c_end) {       // wave-uniform: a wave past the children only stages
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const int off = half * 32 + 8 * s;
+                    for (int s = 0; s < 4 * kChunkSteps; ++s) {
+                        const int off = 64 * (s >> 2) + half * 32 + 8 * (s & 3);
                         const uint4 b0 = *(const uint4 *)(eb + l32 * kEntStride + off);
                         const uint4 b1 = *(const uint4 *)(eb + (32 + l32) * kEntStride + off);
                         acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
@@ -239,11 +267,11 @@ __global__ __launch_bounds__(256) void k_group_topk(
                 if (more) {
                     uint16_t *nb = s_ent[(kc + 1) & 1];
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        *(uint4 *)(nb + (st_row + 32 * j) * kEntStride + st_col) = en[j];
+                    for (int q = 0; q < kEntPieces; ++q)
+                        *(uint4 *)(nb + st_row[q] * kEntStride + st_col[q]) = en[q];
                 }
 #pragma unroll
-                for (int s = 0; s < 4; ++s) a[s] = an[s];
+                for (int s = 0; s < 4 * kChunkSteps; ++s) a[s] = an[s];
                 __syncthreads();
             }
             // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
@@ -324,13 +352,30 @@ __global__ void k_merge_distinct(const int32_t *__restrict__ in_idx,
     }
 }
 
-int n_blocks_persistent(int device)
+// Workgroups of k_group_topk<K> that fit on the device at once (the
+// persistent grid: every one of them is resident from the start)
+int n_blocks_persistent(int device, int K)
 {
+    // one query per device and K (the answer does not change; a race writes
+    // the same value)
+    static int cached[16][4];
+    const int ks = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
+    if (device >= 0 && device < 16 && cached[device][ks] > 0) return cached[device][ks];
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    return 4 * cus;
+    int per_cu = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (K) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<1>, 256, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<2>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<4>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<8>, 256, 0); break;
+    }
+    if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+    if (device >= 0 && device < 16) cached[device][ks] = per_cu * cus;
+    return per_cu * cus;
 }
 
 }  // namespace
@@ -430,7 +475,7 @@ extern "C" int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, in
                  MTIP_ERR_ARG, "mtip_group_topk: null pointer");
     hipStream_t st = (hipStream_t)stream;
     GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
-    const int nb = n_blocks_persistent(device);
+    const int nb = n_blocks_persistent(device, K);
 #define MTIP_GROUP_CASE(KK)                                                                    \
     hipLaunchKernelGGL((k_group_topk<KK>), dim3(nb), dim3(256), 0, st, w.tasks, w.n_tasks,     \
                        w.perm, child_off, child_idx, slices, ld_s, slice_sumsq, images, ld_d,  \
