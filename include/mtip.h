@@ -340,7 +340,8 @@ size_t mtip_match_workspace_size(int64_t N_o, int64_t N_i, int K, int precision)
 
 /*
  * host: orientation rows per GEMM tile for `precision`: 0 restores the
- * built-in rule (256 for BF16 and FP8 with N_o, N_i >= 1024, else 128), 128
+ * built-in rule (256 for BF16 and FP8 when the problem has at least 192
+ * tiles of 256 x 256, else 128), 128
  * or 256 force a tile.  Process-wide; mtip_match_topk reads it once per
  * call, and mtip_match_partial / mtip_topk_merge take the tile explicitly
  * (tile_rows) so that a change between them cannot split a call.  The

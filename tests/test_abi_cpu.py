@@ -69,9 +69,13 @@ def test_long_rows_need_the_128_row_tile():
     lib = _lib.load()
     fake = ctypes.c_void_p(0x1000)    # never dereferenced
     nul = ctypes.c_void_p()
-    n, ld = 1024, 1 << 22
+    n, ld = 4096, 1 << 22
     ws = lib.mtip_match_workspace_size(n, n, 8, 1)
+    # the built-in rule: the 256-row tile from 192 tiles of 256 x 256 (BF16 / FP8)
     assert lib.mtip_gemm_tile(1, n, n) == 256
+    assert lib.mtip_gemm_tile(1, 500, 58642) == 256
+    assert lib.mtip_gemm_tile(1, 1024, 1024) == 128
+    assert lib.mtip_gemm_tile(0, n, n) == 128
     rc = lib.mtip_match_partial(1, fake, ld, nul, fake, n, fake, ld, nul, n, 16, fake, 8, 0, fake,
                                 ws, nul)
     assert rc == _lib.MTIP_ERR_UNSUPPORTED and "128-row tile" in _lib.last_error()

@@ -53,8 +53,9 @@ def test_static_entry_point_equals_workspace_entry_point(precision):
     alpha = torch.ones(1, dtype=torch.float64, device="cuda")
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     K = 1 if precision == "fp32" else 8
-    # first call large enough for the 256-row tile (bf16 / fp8), second smaller
-    for N_o, N_i, N_p, seed in ((3000, 2048, 576, 1), (700, 333, 192, 2)):
+    # first call large enough for the 256-row tile (bf16 / fp8: 12 x 18 tiles
+    # of 256), second smaller
+    for N_o, N_i, N_p, seed in ((3000, 4400, 576, 1), (700, 333, 192, 2)):
         print(f"no-alloc {precision}: N_o={N_o} N_i={N_i} N_p={N_p} K={K}", flush=True)
         Sop, Dop, ld = _operands(torch, engine, N_o, N_i, N_p, precision, seed)
         ws = torch.empty(max(1, engine.workspace_bytes(N_o, N_i, K, precision)),
