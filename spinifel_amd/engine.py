@@ -16,6 +16,7 @@ Pipeline of one ``match`` (all on one stream, inputs resident in HBM):
 6. ``rescore``              -- FP32 and FP64 full distances of the candidates
 """
 import ctypes
+import threading
 from dataclasses import dataclass
 from typing import NamedTuple
 
@@ -566,9 +567,11 @@ def col_mean(x, n_cols=None, stream=None, row_step=1):
 _CENTER_ROWS = 1024
 
 
-# pinned host staging and a copy stream per device for upload_rows
+# pinned host staging and a copy stream per device for upload_rows (the lock
+# serialises callers on different threads: the buffer is shared)
 _STAGING = {}
 _UPLOAD_STREAMS = {}
+_STAGING_LOCK = threading.Lock()
 
 
 def upload_rows(rows, device, chunk_rows=1024):
@@ -586,6 +589,11 @@ def upload_rows(rows, device, chunk_rows=1024):
     dev = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
     arr = np.ascontiguousarray(rows, dtype=np.float32)
     arr = arr.reshape(arr.shape[0], -1)
+    with _STAGING_LOCK:
+        return _upload_rows_locked(arr, dev, chunk_rows)
+
+
+def _upload_rows_locked(arr, dev, chunk_rows):
     n, p = arr.shape
     cur = torch.cuda.current_stream(dev)
     up = _UPLOAD_STREAMS.get(dev)
@@ -618,9 +626,10 @@ def upload_rows(rows, device, chunk_rows=1024):
 
 def release_staging():
     """Free upload_rows' pinned staging buffers (after their uploads finish)."""
-    for st in _STAGING.values():
-        st["done"].synchronize()
-    _STAGING.clear()
+    with _STAGING_LOCK:
+        for st in _STAGING.values():
+            st["done"].synchronize()
+        _STAGING.clear()
 
 
 class ImageSet:
