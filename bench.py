@@ -412,7 +412,8 @@ MODELS = {81: 800, 121: 2700}    # atoms per model size (same atom density)
 def run_batching(args, torch, engine, synth, log):
     """64 batches, sizes log-uniform in [16, 4096], through a 1k / 10k / 100k
     hierarchy, one captured HIP graph per model; agreement with the flat
-    search over the 100k level on an image subset."""
+    search over the 100k level on an image subset.  Then the same pass with
+    the 10k level on e4m3 operands (fp8_level1)."""
     from spinifel_amd import autotune, hierarchy
     from spinifel_amd.quaternions import random_quaternions
     dev = torch.device("cuda", 0)
@@ -478,9 +479,26 @@ def run_batching(args, torch, engine, synth, log):
             agreement_vs_flat_100k=float((got == fl).mean()), flat_subset=int(len(sub)),
             median_angle_to_true_deg=float(np.median(ang(qh[got], true[sub]))),
             flat_median_angle_to_true_deg=float(np.median(ang(qh[fl], true[sub]))))
+        # the same pass with the 10k level's cross terms on e4m3 operands
+        del m, res
+        m8 = hierarchy.HierarchicalMatcher(hm, K=8, fp8_levels=(1,))
+        m8.capture(iset, offs)
+        for _ in range(2):
+            m8.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            res8 = m8.replay()
+        torch.cuda.synchronize()
+        dt8 = (time.perf_counter() - t0) / args.batch_steps
+        got8 = res8["index"][subt].cpu().numpy()
+        out["models"][f"M{M}"]["fp8_level1"] = dict(
+            images_per_s=N_i / dt8, ms_per_pass=1e3 * dt8,
+            agreement_vs_flat_100k=float((got8 == fl).mean()),
+            median_angle_to_true_deg=float(np.median(ang(qh[got8], true[sub]))))
         log("batching", f"M{M}", json.dumps({k: v for k, v in out["models"][f"M{M}"].items()
                                              if k not in ("kernel_ms_eager", "nufft_tune_ms")}))
-        del hm, m, iset, sset, res, flat
+        del hm, m8, res8, iset, sset, flat
         torch.cuda.empty_cache()
     return out
 

@@ -518,6 +518,22 @@ int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, int64_t n_gro
                     int64_t kd, int K, int32_t *top_idx, float *top_val, int device, void *stream);
 
 /*
+ * mtip_group_topk for either operand format: precision MTIP_BF16 (as
+ * mtip_group_topk; the scales are ignored) or MTIP_FP8 (e4m3 rows with
+ * per-row float scales slice_scale (N_fine) and image_scale (N_i), as
+ * mtip_quantize_rows / mtip_residual_rows write them; the cross term is
+ * scale_c scale_e (q_c . q_e) and slice_sumsq holds the squared norms of the
+ * scaled values).  Pitches ld_s / ld_d and kd in elements; rows must be
+ * whole 16-byte units.  No counterpart in the reference (batching form).
+ */
+int mtip_group_topk_prec(int precision, const void *workspace, int n_win, int64_t N_i,
+                         int64_t n_groups, const int32_t *child_off, const int32_t *child_idx,
+                         const void *slices, int64_t ld_s, const float *slice_scale,
+                         const double *slice_sumsq, const void *images, int64_t ld_d,
+                         const float *image_scale, int64_t kd, int K, int32_t *top_idx,
+                         float *top_val, int device, void *stream);
+
+/*
  * host: byte offset in the grouping workspace of the int32 entry permutation
  * (N_e entries: each group's entries together, groups in order, entries
  * with no valid winner last), valid after mtip_group_by_winner; usable as

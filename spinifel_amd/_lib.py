@@ -71,6 +71,8 @@ _SIGS = {
     "mtip_group_by_winner": (_i32, [_vp, _i64, _i32, _i64, _i64, _vp, _c.c_size_t, _vp]),
     "mtip_group_topk": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64,
                                _i32, _vp, _vp, _i32, _vp]),
+    "mtip_group_topk_prec": (_i32, [_i32, _vp, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp,
+                                    _vp, _i64, _vp, _i64, _i32, _vp, _vp, _i32, _vp]),
     "mtip_group_bad_count": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp]),
     "mtip_group_perm_offset": (_c.c_size_t, [_i64, _i64]),
     "mtip_merge_distinct": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp]),

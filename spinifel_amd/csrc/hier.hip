@@ -24,6 +24,7 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
 constexpr int kTaskImages = 64;     // entries per task (two MFMA blocks of columns)
 constexpr uint64_t KEY_NONE = ~0ull;
@@ -133,61 +134,75 @@ __global__ void k_tasks(const int32_t *__restrict__ counts, const int32_t *__res
         tasks[t] = make_int4((int32_t)g, b + s, b + min(n, s + kTaskImages), 0);
 }
 
-__device__ __forceinline__ uint4 ld16(const uint16_t *row, int64_t e)
-{
-    return row ? *(const uint4 *)(row + e) : make_uint4(0, 0, 0, 0);
-}
-
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 {
     return __builtin_bit_cast(bf16x8, v);
 }
 
+__device__ __forceinline__ intx8 as_intx8(uint4 lo, uint4 hi)
+{
+    return intx8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+}
+
 // Per task: the K children of the group with the smallest
 //   v = ||s_c||^2 - 2 s_c . d_e
 // for each of up to 64 entries (the centred, scaled operands of the dense
-// path, so v ranks like the squared distance).  kd: pixels summed, a
-// multiple of 64 within both row pitches; columns past N_p are zero.
+// path, so v ranks like the squared distance).  BF16 operands, or e4m3 with
+// per-row scales (then s_c . d_e = scale_c scale_e (q_c . q_e)).  kd_b:
+// bytes summed per row, a multiple of 64 within both row pitches; columns
+// past N_p are zero.
 //
 // Wave w takes children c0 + 32 w .. c0 + 32 w + 31 of each pass of 128,
 // for two 32-entry column blocks, reading its children's rows straight from
 // memory (each child row once per task).  The 64 entries' rows are shared
-// by the four waves: each chunk of kChunk pixels of them is staged once in
+// by the four waves: each chunk of kChunkBytes of them is staged once in
 // LDS (double-buffered), so an entry row is read from memory once per pass
 // instead of once per wave.  A chunk's loads (children and entries) are
 // issued a whole chunk ahead of their MFMAs: the kernel is bound by that
-// memory latency, so a chunk is two 64-pixel MFMA steps long.  The waves'
-// running top-K lists are merged in LDS at the end.
+// memory latency, so a chunk is 256 bytes of each row (two BF16 or four e4m3
+// MFMA steps).  The waves' running top-K lists are merged in LDS at the end.
 constexpr int kXcdsH = 8;                      // XCDs of the device (workgroups dealt round robin)
-constexpr int kChunkSteps = 2;                 // 64-pixel MFMA steps per chunk
-constexpr int kChunk = 64 * kChunkSteps;       // pixels per chunk
-constexpr int kEntStride = kChunk + 8;         // bf16 per staged entry row (16-byte multiple)
-constexpr int kEntPieces = kTaskImages * kChunk / 8 / 256;   // 16-byte pieces per thread
+constexpr int kChunkBytes = 256;               // bytes of each row per chunk
+constexpr int kEntStrideB = kChunkBytes + 16;  // bytes per staged entry row
+constexpr int kEntPieces = kTaskImages * kChunkBytes / 16 / 256;   // 16-byte pieces per thread
 
-template <int K>
+__device__ __forceinline__ uint4 ld16b(const uint8_t *row, int64_t e)
+{
+    return row ? *(const uint4 *)(row + e) : make_uint4(0, 0, 0, 0);
+}
+
+template <int PREC, int K>
 __global__ __launch_bounds__(256) void k_group_topk(
     const int4 *__restrict__ tasks, const int32_t *__restrict__ n_tasks,
     const int32_t *__restrict__ perm, const int32_t *__restrict__ child_off,
-    const int32_t *__restrict__ child_idx, const uint16_t *__restrict__ S, int64_t lds,
-    const double *__restrict__ s_sumsq, const uint16_t *__restrict__ D, int64_t ldd, int64_t kd,
-    int n_win, int32_t *__restrict__ out_idx, float *__restrict__ out_val)
+    const int32_t *__restrict__ child_idx, const uint8_t *__restrict__ S, int64_t lds_b,
+    const float *__restrict__ s_scale, const double *__restrict__ s_sumsq,
+    const uint8_t *__restrict__ D, int64_t ldd_b, const float *__restrict__ d_scale,
+    int64_t kd_b, int n_win, int32_t *__restrict__ out_idx, float *__restrict__ out_val)
 {
     __shared__ uint64_t s_top[4][kTaskImages][K];
-    __shared__ __attribute__((aligned(16))) uint16_t s_ent[2][kTaskImages * kEntStride];
+    __shared__ __attribute__((aligned(16))) uint8_t s_ent[2][kTaskImages * kEntStrideB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
     const int nt = n_tasks[0];
-    // staging: piece q of thread tid is entry row (tid + 256 q) / (kChunk / 8),
-    // 8-element column ((tid + 256 q) % (kChunk / 8)) * 8 of a chunk
-    constexpr int kRowPieces = kChunk / 8;
+    // staging: piece q of thread tid is entry row (tid + 256 q) / 16, bytes
+    // 16 ((tid + 256 q) % 16) .. + 15 of a chunk
+    constexpr int kRowPieces = kChunkBytes / 16;
     int st_row[kEntPieces], st_col[kEntPieces];
 #pragma unroll
     for (int q = 0; q < kEntPieces; ++q) {
         st_row[q] = (tid + 256 * q) / kRowPieces;
-        st_col[q] = ((tid + 256 * q) % kRowPieces) * 8;
+        st_col[q] = ((tid + 256 * q) % kRowPieces) * 16;
     }
-    // kd is a multiple of 64; a last chunk of one step reads zeros past kd
-    const int64_t nk = (kd + kChunk - 1) / kChunk;
+    // the byte offset of this lane's s-th 16-byte piece of a child row in a
+    // chunk: BF16, two 128-byte MFMA steps, each lane half 64 bytes in four
+    // pieces; e4m3, four 64-byte steps, each lane half 32 bytes in two
+    auto a_off = [&](int s) __attribute__((always_inline)) {
+        return PREC == MTIP_BF16 ? 128 * (s >> 2) + 64 * half + 16 * (s & 3)
+                                 : 64 * (s >> 1) + 32 * half + 16 * (s & 1);
+    };
+    // kd_b is a multiple of 64; a chunk past it reads zeros
+    const int64_t nk = (kd_b + kChunkBytes - 1) / kChunkBytes;
 
     // workgroup b runs on XCD b % 8: XCD x takes the x-th of 8 contiguous
     // task ranges, dealt round robin to its workgroups, so the tasks of one
@@ -201,11 +216,20 @@ __global__ __launch_bounds__(256) void k_group_topk(
         const int4 tk = tasks[task];
         const int g = tk.x, p0 = tk.y, p1 = tk.z;
         const int c_begin = child_off[g], c_end = child_off[g + 1];
-        const uint16_t *erow[kEntPieces];
+        const uint8_t *erow[kEntPieces];
 #pragma unroll
         for (int q = 0; q < kEntPieces; ++q) {
             const int pe = p0 + st_row[q];
-            erow[q] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd : nullptr;
+            erow[q] = pe < p1 ? D + (int64_t)(perm[pe] / n_win) * ldd_b : nullptr;
+        }
+        // e4m3: the scales of this lane's two entry columns
+        float esc[2] = {1.0f, 1.0f};
+        if constexpr (PREC == MTIP_FP8) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int pe = p0 + 32 * j + l32;
+                esc[j] = pe < p1 ? d_scale[perm[pe] / n_win] : 0.0f;
+            }
         }
         uint64_t top[2][K];
 #pragma unroll
@@ -213,65 +237,79 @@ __global__ __launch_bounds__(256) void k_group_topk(
 #pragma unroll
             for (int r = 0; r < K; ++r) top[j][r] = KEY_NONE;
 
-        // element e of a row, or zero past kd (or for a missing row)
-        auto ldk = [&](const uint16_t *row, int64_t e) __attribute__((always_inline)) {
-            return e < kd ? ld16(row, e) : make_uint4(0, 0, 0, 0);
+        // bytes e.. of a row, or zero past kd_b (or for a missing row)
+        auto ldk = [&](const uint8_t *row, int64_t e) __attribute__((always_inline)) {
+            return e < kd_b ? ld16b(row, e) : make_uint4(0, 0, 0, 0);
         };
         for (int cp = c_begin; cp < c_end; cp += 128) {
             const int c0 = cp + 32 * wave;
             const int crow = c0 + l32;
             const int child = crow < c_end ? child_idx[crow] : -1;
-            const uint16_t *srow = child >= 0 ? S + (int64_t)child * lds : nullptr;
+            const uint8_t *srow = child >= 0 ? S + (int64_t)child * lds_b : nullptr;
             floatx16 acc[2];
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[j][v] = 0.0f;
             // chunk 0: entries into LDS buffer 0, this wave's children into registers
-            uint4 a[4 * kChunkSteps], en[kEntPieces];
+            uint4 a[8], en[kEntPieces];
 #pragma unroll
             for (int q = 0; q < kEntPieces; ++q) en[q] = ldk(erow[q], st_col[q]);
 #pragma unroll
             for (int q = 0; q < kEntPieces; ++q)
-                *(uint4 *)(s_ent[0] + st_row[q] * kEntStride + st_col[q]) = en[q];
+                *(uint4 *)(s_ent[0] + st_row[q] * kEntStrideB + st_col[q]) = en[q];
 #pragma unroll
-            for (int s = 0; s < 4 * kChunkSteps; ++s)
-                a[s] = ldk(srow, 64 * (s >> 2) + half * 32 + 8 * (s & 3));
+            for (int s = 0; s < 8; ++s) a[s] = ldk(srow, a_off(s));
             __syncthreads();
             for (int64_t kc = 0; kc < nk; ++kc) {
                 const bool more = kc + 1 < nk;
-                const int64_t e1 = (kc + 1) * kChunk;
-                uint4 an[4 * kChunkSteps];
+                const int64_t e1 = (kc + 1) * kChunkBytes;
+                uint4 an[8];
 #pragma unroll
                 for (int q = 0; q < kEntPieces; ++q)
                     en[q] = more ? ldk(erow[q], e1 + st_col[q]) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int s = 0; s < 4 * kChunkSteps; ++s)
-                    an[s] = more ? ldk(srow, e1 + 64 * (s >> 2) + half * 32 + 8 * (s & 3))
-                                 : make_uint4(0, 0, 0, 0);
-                const uint16_t *eb = s_ent[kc & 1];
+                for (int s = 0; s < 8; ++s)
+                    an[s] = more ? ldk(srow, e1 + a_off(s)) : make_uint4(0, 0, 0, 0);
+                const uint8_t *eb = s_ent[kc & 1];
                 if (c0 < c_end) {       // wave-uniform: a wave past the children only stages
+                    if constexpr (PREC == MTIP_BF16) {
 #pragma unroll
-                    for (int s = 0; s < 4 * kChunkSteps; ++s) {
-                        const int off = 64 * (s >> 2) + half * 32 + 8 * (s & 3);
-                        const uint4 b0 = *(const uint4 *)(eb + l32 * kEntStride + off);
-                        const uint4 b1 = *(const uint4 *)(eb + (32 + l32) * kEntStride + off);
-                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            as_bf16x8(a[s]), as_bf16x8(b0), acc[0], 0, 0, 0);
-                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            as_bf16x8(a[s]), as_bf16x8(b1), acc[1], 0, 0, 0);
+                        for (int s = 0; s < 8; ++s) {
+                            const int off = a_off(s);
+                            const uint4 b0 = *(const uint4 *)(eb + l32 * kEntStrideB + off);
+                            const uint4 b1 = *(const uint4 *)(eb + (32 + l32) * kEntStrideB + off);
+                            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                as_bf16x8(a[s]), as_bf16x8(b0), acc[0], 0, 0, 0);
+                            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                as_bf16x8(a[s]), as_bf16x8(b1), acc[1], 0, 0, 0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int off = a_off(2 * t);
+                            const intx8 av = as_intx8(a[2 * t], a[2 * t + 1]);
+                            const uint8_t *r0 = eb + l32 * kEntStrideB + off;
+                            const uint8_t *r1 = eb + (32 + l32) * kEntStrideB + off;
+                            const intx8 b0 = as_intx8(*(const uint4 *)r0, *(const uint4 *)(r0 + 16));
+                            const intx8 b1 = as_intx8(*(const uint4 *)r1, *(const uint4 *)(r1 + 16));
+                            acc[0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                av, b0, acc[0], 0, 0, 0, 127, 0, 127);
+                            acc[1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                av, b1, acc[1], 0, 0, 0, 127, 0, 127);
+                        }
                     }
                 }
                 // the other buffer was last read in chunk kc - 1, before the
                 // barrier that ended it
                 if (more) {
-                    uint16_t *nb = s_ent[(kc + 1) & 1];
+                    uint8_t *nb = s_ent[(kc + 1) & 1];
 #pragma unroll
                     for (int q = 0; q < kEntPieces; ++q)
-                        *(uint4 *)(nb + st_row[q] * kEntStride + st_col[q]) = en[q];
+                        *(uint4 *)(nb + st_row[q] * kEntStrideB + st_col[q]) = en[q];
                 }
 #pragma unroll
-                for (int s = 0; s < 4 * kChunkSteps; ++s) a[s] = an[s];
+                for (int s = 0; s < 8; ++s) a[s] = an[s];
                 __syncthreads();
             }
             // rows of this lane: child c0 + 8 (v / 4) + 4 half + v % 4
@@ -281,10 +319,13 @@ __global__ __launch_bounds__(256) void k_group_topk(
                 if (r < c_end) {
                     const int cid = child_idx[r];
                     const float ss = (float)s_sumsq[cid];
+                    const float csc = PREC == MTIP_FP8 ? s_scale[cid] : 1.0f;
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        mtip::topk_insert<K>(top[j], mtip::make_key(fmaf(-2.0f, acc[j][v], ss),
+                    for (int j = 0; j < 2; ++j) {
+                        const float dot = PREC == MTIP_FP8 ? acc[j][v] * (csc * esc[j]) : acc[j][v];
+                        mtip::topk_insert<K>(top[j], mtip::make_key(fmaf(-2.0f, dot, ss),
                                                                     (uint32_t)cid));
+                    }
                 }
             }
         }
@@ -354,10 +395,11 @@ __global__ void k_merge_distinct(const int32_t *__restrict__ in_idx,
 
 // Workgroups of k_group_topk<K> that fit on the device at once (the
 // persistent grid: every one of them is resident from the start)
+template <int PREC>
 int n_blocks_persistent(int device, int K)
 {
-    // one query per device and K (the answer does not change; a race writes
-    // the same value)
+    // one query per device, precision and K (the answer does not change; a
+    // race writes the same value)
     static int cached[16][4];
     const int ks = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
     if (device >= 0 && device < 16 && cached[device][ks] > 0) return cached[device][ks];
@@ -368,10 +410,10 @@ int n_blocks_persistent(int device, int K)
     int per_cu = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (K) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<1>, 256, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<2>, 256, 0); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<4>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<8>, 256, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<PREC, 1>, 256, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<PREC, 2>, 256, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<PREC, 4>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_topk<PREC, 8>, 256, 0); break;
     }
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     if (device >= 0 && device < 16) cached[device][ks] = per_cu * cus;
@@ -455,40 +497,85 @@ extern "C" int mtip_group_by_winner(const int32_t *winners, int64_t win_stride, 
     return MTIP_OK;
 }
 
+namespace {
+
+int group_topk(const char *fn, int precision, const void *workspace, int n_win, int64_t N_i,
+               int64_t n_groups, const int32_t *child_off, const int32_t *child_idx,
+               const void *slices, int64_t ld_s, const float *slice_scale,
+               const double *slice_sumsq, const void *images, int64_t ld_d,
+               const float *image_scale, int64_t kd, int K, int32_t *top_idx, float *top_val,
+               int device, void *stream)
+{
+    MTIP_REQUIRE(precision == MTIP_BF16 || precision == MTIP_FP8, MTIP_ERR_ARG,
+                 std::string(fn) + ": precision must be BF16 or FP8");
+    MTIP_REQUIRE(N_i >= 0 && n_groups > 0 && n_win >= 1, MTIP_ERR_ARG,
+                 std::string(fn) + ": bad sizes");
+    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG,
+                 std::string(fn) + ": K must be 1, 2, 4 or 8");
+    MTIP_REQUIRE(kd > 0 && kd % 64 == 0, MTIP_ERR_ARG,
+                 std::string(fn) + ": kd must be a positive multiple of 64");
+    const int eb = precision == MTIP_FP8 ? 1 : 2;      // bytes per element
+    MTIP_REQUIRE(ld_s >= kd && ld_d >= kd && (ld_s * eb) % 16 == 0 && (ld_d * eb) % 16 == 0,
+                 MTIP_ERR_ARG,
+                 std::string(fn) + ": row pitches must be >= kd and whole 16-byte units");
+    if (N_i == 0) return MTIP_OK;
+    MTIP_REQUIRE(workspace && child_off && child_idx && slices && slice_sumsq && images &&
+                     top_idx && top_val,
+                 MTIP_ERR_ARG, std::string(fn) + ": null pointer");
+    MTIP_REQUIRE(precision != MTIP_FP8 || (slice_scale && image_scale), MTIP_ERR_ARG,
+                 std::string(fn) + ": e4m3 operands need their row scales");
+    hipStream_t st = (hipStream_t)stream;
+    GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
+#define MTIP_GROUP_CASE(PP, KK)                                                                \
+    hipLaunchKernelGGL((k_group_topk<PP, KK>), dim3(n_blocks_persistent<PP>(device, KK)),    \
+                       dim3(256), 0, st, w.tasks, w.n_tasks, w.perm, child_off, child_idx,   \
+                       (const uint8_t *)slices, ld_s * eb, slice_scale, slice_sumsq,          \
+                       (const uint8_t *)images, ld_d * eb, image_scale, kd * eb, n_win,       \
+                       top_idx, top_val)
+    if (precision == MTIP_BF16) {
+        switch (K) {
+        case 1: MTIP_GROUP_CASE(MTIP_BF16, 1); break;
+        case 2: MTIP_GROUP_CASE(MTIP_BF16, 2); break;
+        case 4: MTIP_GROUP_CASE(MTIP_BF16, 4); break;
+        default: MTIP_GROUP_CASE(MTIP_BF16, 8); break;
+        }
+    } else {
+        switch (K) {
+        case 1: MTIP_GROUP_CASE(MTIP_FP8, 1); break;
+        case 2: MTIP_GROUP_CASE(MTIP_FP8, 2); break;
+        case 4: MTIP_GROUP_CASE(MTIP_FP8, 4); break;
+        default: MTIP_GROUP_CASE(MTIP_FP8, 8); break;
+        }
+    }
+#undef MTIP_GROUP_CASE
+    MTIP_HIP_CHECK(hipGetLastError());
+    return MTIP_OK;
+}
+
+}  // namespace
+
 extern "C" int mtip_group_topk(const void *workspace, int n_win, int64_t N_i, int64_t n_groups,
                                const int32_t *child_off, const int32_t *child_idx,
                                const uint16_t *slices, int64_t ld_s, const double *slice_sumsq,
                                const uint16_t *images, int64_t ld_d, int64_t kd, int K,
                                int32_t *top_idx, float *top_val, int device, void *stream)
 {
-    MTIP_REQUIRE(N_i >= 0 && n_groups > 0 && n_win >= 1, MTIP_ERR_ARG,
-                 "mtip_group_topk: bad sizes");
-    MTIP_REQUIRE(K == 1 || K == 2 || K == 4 || K == 8, MTIP_ERR_ARG,
-                 "mtip_group_topk: K must be 1, 2, 4 or 8");
-    MTIP_REQUIRE(kd > 0 && kd % 64 == 0, MTIP_ERR_ARG,
-                 "mtip_group_topk: kd must be a positive multiple of 64");
-    MTIP_REQUIRE(ld_s >= kd && ld_d >= kd && ld_s % 8 == 0 && ld_d % 8 == 0, MTIP_ERR_ARG,
-                 "mtip_group_topk: row pitches must be >= kd and multiples of 8");
-    if (N_i == 0) return MTIP_OK;
-    MTIP_REQUIRE(workspace && child_off && child_idx && slices && slice_sumsq && images &&
-                     top_idx && top_val,
-                 MTIP_ERR_ARG, "mtip_group_topk: null pointer");
-    hipStream_t st = (hipStream_t)stream;
-    GroupWs w = carve(const_cast<void *>(workspace), N_i * n_win, n_groups);
-    const int nb = n_blocks_persistent(device, K);
-#define MTIP_GROUP_CASE(KK)                                                                    \
-    hipLaunchKernelGGL((k_group_topk<KK>), dim3(nb), dim3(256), 0, st, w.tasks, w.n_tasks,     \
-                       w.perm, child_off, child_idx, slices, ld_s, slice_sumsq, images, ld_d,  \
-                       kd, n_win, top_idx, top_val)
-    switch (K) {
-    case 1: MTIP_GROUP_CASE(1); break;
-    case 2: MTIP_GROUP_CASE(2); break;
-    case 4: MTIP_GROUP_CASE(4); break;
-    default: MTIP_GROUP_CASE(8); break;
-    }
-#undef MTIP_GROUP_CASE
-    MTIP_HIP_CHECK(hipGetLastError());
-    return MTIP_OK;
+    return group_topk("mtip_group_topk", MTIP_BF16, workspace, n_win, N_i, n_groups, child_off,
+                      child_idx, slices, ld_s, nullptr, slice_sumsq, images, ld_d, nullptr, kd, K,
+                      top_idx, top_val, device, stream);
+}
+
+extern "C" int mtip_group_topk_prec(int precision, const void *workspace, int n_win, int64_t N_i,
+                                    int64_t n_groups, const int32_t *child_off,
+                                    const int32_t *child_idx, const void *slices, int64_t ld_s,
+                                    const float *slice_scale, const double *slice_sumsq,
+                                    const void *images, int64_t ld_d, const float *image_scale,
+                                    int64_t kd, int K, int32_t *top_idx, float *top_val,
+                                    int device, void *stream)
+{
+    return group_topk("mtip_group_topk_prec", precision, workspace, n_win, N_i, n_groups,
+                      child_off, child_idx, slices, ld_s, slice_scale, slice_sumsq, images, ld_d,
+                      image_scale, kd, K, top_idx, top_val, device, stream);
 }
 
 extern "C" int mtip_group_bad_count(const void *workspace, int n_win, int64_t N_i,

@@ -180,7 +180,16 @@ class HierarchyModel:
         self.center = self.levels[0].center
         self.ops = [engine.residual(sl.f32, "bf16", sl.N_p, center=self.center, stream=stream)
                     for sl in self.levels]
+        self._ops8 = {}
         self.bad = sum(int(sl.bad.item()) for sl in self.levels)
+
+    def op8(self, level, stream=None):
+        """The level's centred residual as e4m3 rows with per-row scales (built once)."""
+        if level not in self._ops8:
+            sl = self.levels[level]
+            self._ops8[level] = engine.residual(sl.f32, "fp8", sl.N_p, center=self.center,
+                                                stream=stream)
+        return self._ops8[level]
 
 
 class HierarchicalMatcher:
@@ -188,11 +197,21 @@ class HierarchicalMatcher:
 
     parts / k0: level 0 is searched as ``parts`` contiguous (Morton) ranges
     with the top ``k0`` of each; kl_mid: children kept per entry on middle
-    levels; K: candidates of the last level rescored in FP32 / FP64."""
+    levels; K: candidates of the last level rescored in FP32 / FP64;
+    fp8_levels: grouped levels (1 .. n_levels - 1) whose cross terms run on
+    e4m3 operands with per-row scales instead of bf16 (the images' e4m3
+    residual is built once per pass).  At 4 x 32 x 128 px and 1k / 10k /
+    100k levels, e4m3 on the 10k level left the agreement with the flat
+    search within 0.03 % (M = 81 and 121); on the 100k level it fell from
+    99.1 to 97.1 % at M = 121, so the default keeps bf16 everywhere."""
 
-    def __init__(self, model, K=8, parts=2, k0=8, kl_mid=4):
+    def __init__(self, model, K=8, parts=2, k0=8, kl_mid=4, fp8_levels=()):
         if K not in (1, 2, 4, 8) or k0 not in (1, 2, 4, 8) or kl_mid not in (1, 2, 4, 8):
             raise ValueError("K, k0 and kl_mid must be 1, 2, 4 or 8")
+        fp8_levels = tuple(sorted(set(int(l) for l in fp8_levels)))
+        if any(l < 1 or l >= model.h.n_levels for l in fp8_levels):
+            raise ValueError("fp8_levels must be grouped levels 1 .. n_levels - 1")
+        self.fp8_levels = fp8_levels
         n0 = model.levels[0].N_o
         if parts < 1 or n0 // parts < k0:
             raise ValueError(f"{parts} parts of {n0} level-0 orientations cannot each give {k0}")
@@ -216,6 +235,7 @@ class HierarchicalMatcher:
                 alpha = engine.segment_scaling_ratio(images.row_sum, images.row_sumsq, offsets,
                                                      L0.row_sum, L0.row_sumsq, N_p, stream)
             D = images.operand("bf16", alpha, m.center, stream)
+            D8 = images.operand("fp8", alpha, m.center, stream) if self.fp8_levels else None
         last = h.n_levels - 1
         one = engine._one(dev)
         op0 = m.ops[0]
@@ -256,12 +276,22 @@ class HierarchicalMatcher:
             eidx = torch.full((N_i * T, kl), -1, dtype=torch.int32, device=dev)
             evals = torch.full((N_i * T, kl), float("inf"), dtype=torch.float32, device=dev)
             off, ch = h.children_dev[l - 1]
-            op = m.ops[l]
             with engine._span(timers, f"level{l}"):
-                _lib.check(lib.mtip_group_topk(_ptr(ws), T, N_i, n_groups, _ptr(off), _ptr(ch),
-                                               _ptr(op.data), op.data.shape[1], _ptr(op.sumsq),
-                                               _ptr(D.data), ld, ld, kl, _ptr(eidx), _ptr(evals),
-                                               dev.index or 0, st), "mtip_group_topk")
+                if l in self.fp8_levels:
+                    op = m.op8(l, stream)
+                    ld8 = D8.data.shape[1]
+                    _lib.check(lib.mtip_group_topk_prec(
+                        _lib.MTIP_FP8, _ptr(ws), T, N_i, n_groups, _ptr(off), _ptr(ch),
+                        _ptr(op.data), op.data.shape[1], _ptr(op.scale), _ptr(op.sumsq),
+                        _ptr(D8.data), ld8, _ptr(D8.scale), ld8, kl, _ptr(eidx), _ptr(evals),
+                        dev.index or 0, st), "mtip_group_topk_prec")
+                else:
+                    op = m.ops[l]
+                    _lib.check(lib.mtip_group_topk(_ptr(ws), T, N_i, n_groups, _ptr(off),
+                                                   _ptr(ch), _ptr(op.data), op.data.shape[1],
+                                                   _ptr(op.sumsq), _ptr(D.data), ld, ld, kl,
+                                                   _ptr(eidx), _ptr(evals), dev.index or 0, st),
+                               "mtip_group_topk")
             if l == last:
                 idx = torch.empty((N_i, kl), dtype=torch.int32, device=dev)
                 val = torch.empty((N_i, kl), dtype=torch.float32, device=dev)

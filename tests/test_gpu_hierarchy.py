@@ -86,6 +86,71 @@ def test_group_topk_equals_brute_force():
     assert checked > N_i // 2
 
 
+def test_group_topk_e4m3_equals_brute_force():
+    """mtip_group_topk_prec on e4m3 rows with per-row scales against FP64 sums
+    of the dequantised values; the BF16 entry point and the precision one
+    agree bitwise on bf16 operands."""
+    rng = np.random.default_rng(1)
+    dev = torch.device("cuda", 0)
+    n_c, n_f, N_i, P, K = 19, 600, 257, 448, 8
+    S32 = rng.standard_normal((n_f, P)).astype(np.float32)
+    D32 = rng.standard_normal((N_i, P)).astype(np.float32) * 3.0
+    ld = engine.padded(P)
+    Sd = torch.zeros((n_f, ld), dtype=torch.float32, device=dev)
+    Sd[:, :P] = torch.as_tensor(S32)
+    Dd = torch.zeros((N_i, ld), dtype=torch.float32, device=dev)
+    Dd[:, :P] = torch.as_tensor(D32)
+    So, Do = engine.residual(Sd, "fp8", P), engine.residual(Dd, "fp8", P)
+    Sb, Db = engine.residual(Sd, "bf16", P), engine.residual(Dd, "bf16", P)
+    kids = [np.sort(rng.choice(n_f, rng.integers(0, 150), replace=False)) for _ in range(n_c)]
+    off = np.concatenate([[0], np.cumsum([len(k) for k in kids])]).astype(np.int32)
+    ch = np.concatenate(kids).astype(np.int32)
+    win = rng.integers(0, n_c, N_i).astype(np.int32)
+    lib = _lib.load()
+    nb = int(lib.mtip_group_workspace_size(N_i, n_c))
+    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+    st = engine._stream()
+    _lib.check(lib.mtip_group_by_winner(_ptr(torch.as_tensor(win).to(dev)), 1, 1, N_i, n_c,
+                                        _ptr(ws), nb, st), "group")
+    offd, chd = torch.as_tensor(off).to(dev), torch.as_tensor(ch).to(dev)
+    out = {}
+    for name, prec, S, D in (("fp8", _lib.MTIP_FP8, So, Do), ("bf16", _lib.MTIP_BF16, Sb, Db)):
+        idx = torch.full((N_i, K), -7, dtype=torch.int32, device=dev)
+        val = torch.zeros((N_i, K), dtype=torch.float32, device=dev)
+        _lib.check(lib.mtip_group_topk_prec(prec, _ptr(ws), 1, N_i, n_c, _ptr(offd), _ptr(chd),
+                                            _ptr(S.data), ld, _ptr(S.scale), _ptr(S.sumsq),
+                                            _ptr(D.data), ld, _ptr(D.scale), ld, K, _ptr(idx),
+                                            _ptr(val), 0, st), "group_topk_prec")
+        out[name] = (idx, val)
+    idx_b = torch.full((N_i, K), -7, dtype=torch.int32, device=dev)
+    val_b = torch.zeros((N_i, K), dtype=torch.float32, device=dev)
+    _lib.check(lib.mtip_group_topk(_ptr(ws), 1, N_i, n_c, _ptr(offd), _ptr(chd), _ptr(Sb.data), ld,
+                                   _ptr(Sb.sumsq), _ptr(Db.data), ld, ld, K, _ptr(idx_b),
+                                   _ptr(val_b), 0, st), "group_topk")
+    torch.cuda.synchronize()
+    assert torch.equal(idx_b, out["bf16"][0]) and torch.equal(val_b, out["bf16"][1])
+    # FP64 brute force on the dequantised e4m3 values
+    sdq = (So.data[:, :P].view(torch.float8_e4m3fn).double() * So.scale.double()[:, None]).cpu().numpy()
+    ddq = (Do.data[:, :P].view(torch.float8_e4m3fn).double() * Do.scale.double()[:, None]).cpu().numpy()
+    ss = So.sumsq.cpu().numpy()
+    np.testing.assert_allclose(ss, (sdq ** 2).sum(1), rtol=1e-12)
+    idx, val = (t.cpu().numpy() for t in out["fp8"])
+    checked = 0
+    for i in range(N_i):
+        c = kids[win[i]]
+        v = ss[c] - 2.0 * (sdq[c] @ ddq[i])
+        order = np.lexsort((c, v))[:K]
+        n = len(order)
+        if n:
+            np.testing.assert_allclose(val[i, :n], v[order], rtol=2e-5, atol=1e-4 * P)
+            gaps = np.diff(np.sort(v)[:K + 1]) if len(v) > K else np.diff(np.sort(v))
+            if len(gaps) == 0 or gaps.min() > 1e-4 * np.abs(v).max():
+                np.testing.assert_array_equal(idx[i, :n], c[order])
+                checked += 1
+        assert (idx[i, n:] == -1).all() and np.isinf(val[i, n:]).all()
+    assert checked > N_i // 2
+
+
 def test_beam_grouping_and_distinct_merge_equal_brute_force():
     rng = np.random.default_rng(4)
     dev = torch.device("cuda", 0)
@@ -188,7 +253,8 @@ def test_hierarchy_finds_planted_and_agrees_with_flat(small_hier):
         assert np.all(a[b0:b1] == a[b0]) and abs(a[b0] / sc - 1.0) < 0.3
 
 
-def test_hierarchy_noisy_images_agree_with_flat(small_hier):
+@pytest.mark.parametrize("fp8_levels", [(), (1,)])
+def test_hierarchy_noisy_images_agree_with_flat(small_hier, fp8_levels):
     hm, ac, ppr, E = small_hier["hm"], small_hier["ac"], small_hier["ppr"], small_hier["E"]
     rng = np.random.default_rng(2)
     true = synth.random_quaternions(300, rng)
@@ -196,20 +262,21 @@ def test_hierarchy_noisy_images_agree_with_flat(small_hier):
     imgs = synth.poisson_images(sl, 5.0, rng)
     iset = engine.ImageSet(imgs.reshape(len(imgs), -1))
     offs = [0, 100, 300]
-    r = hierarchy.HierarchicalMatcher(hm, K=8).run(iset, offs)
+    r = hierarchy.HierarchicalMatcher(hm, K=8, fp8_levels=fp8_levels).run(iset, offs)
     flat = engine.match_resident(hm.levels[-1], iset, "bf16", K=8, alpha=r["alpha"])
     agree = (r["index"].cpu().numpy() == flat["index"].cpu().numpy()).mean()
     assert agree >= 0.85
 
 
-def test_graph_replay_equals_eager(small_hier):
+@pytest.mark.parametrize("fp8_levels", [(), (1,)])
+def test_graph_replay_equals_eager(small_hier, fp8_levels):
     hm = small_hier["hm"]
     rng = np.random.default_rng(3)
     _, imgs = _planted(hm, 257, rng)
     imgs = imgs + torch.rand_like(imgs)
     iset = engine.ImageSet(imgs)
     offs = [0, 17, 100, 257]
-    m = hierarchy.HierarchicalMatcher(hm, K=8)
+    m = hierarchy.HierarchicalMatcher(hm, K=8, fp8_levels=fp8_levels)
     eager = m.run(iset, offs)
     e_idx, e_d = eager["index"].cpu().numpy(), eager["best_d64"].cpu().numpy()
     m.capture(iset, offs)
