@@ -160,7 +160,10 @@ class Server:
             if images.device.type != "cpu" or not images.is_pinned() or \
                     images.dtype != torch.float32:
                 raise ValueError("tensor requests must be pinned host float32 tensors")
-            images = images.reshape(images.shape[0], -1).contiguous()
+            images = images.reshape(images.shape[0], -1)
+            if not images.is_contiguous():
+                # a contiguous copy is not pinned by itself
+                images = images.contiguous().pin_memory()
         else:
             images = np.ascontiguousarray(images, np.float32).reshape(len(images), -1)
         if images.shape[1] != self.models[model].N_p:
