@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MTIP_ABI_VERSION 4
+#define MTIP_ABI_VERSION 5
 
 enum {
     MTIP_OK = 0,
@@ -399,6 +399,28 @@ int mtip_match_partial(int precision, const void *slices, int64_t ld_s, const fl
                        void *stream);
 int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i, int K,
                     int tile_rows, int32_t *top_idx, float *top_val, void *stream);
+
+/*
+ * mtip_match_partial with flags.  MTIP_MATCH_SPLIT_EDGES: on the 256-row
+ * tile, the thin edge tiles (a last row of orientation tiles or column of
+ * image tiles with at most 64 valid rows) run as a short second launch, each
+ * split in parts of K whose FP32 sums are added in a fixed order, after the
+ * whole tiles: a thin tile takes as long as a whole one, so without the split
+ * they can add a nearly empty round of tiles (10,000 x 10,000: 1,600 tiles on
+ * 256 CUs, 79 of them thin).  The split applies when its partial sums fit in
+ * mtip_match_workspace_size(N_o, N_i, K, precision) bytes beside the keys, so
+ * whether it applies depends on the sizes only; the workspace must then hold
+ * that size.  Keys of the split tiles can differ from mtip_match_partial's in
+ * the last bits (another summation order); pass 0 for exactly its results
+ * (callers that need one summation order whatever the batch, e.g. serving).
+ * No counterpart in the reference (one GEMM path there).
+ */
+#define MTIP_MATCH_SPLIT_EDGES 1
+int mtip_match_partial_ex(int precision, const void *slices, int64_t ld_s,
+                          const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                          const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                          int64_t N_p, const double *alpha, int K, int tile_rows, int flags,
+                          void *workspace, size_t workspace_bytes, void *stream);
 
 /*
  * Bytes of the device scratch mtip_match_topk_static uses (32 MiB), one

@@ -81,6 +81,8 @@ _SIGS = {
     "mtip_gemm_tile": (_i32, [_i32, _i64, _i64]),
     "mtip_match_partial": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
                                   _vp, _i32, _i32, _vp, _c.c_size_t, _vp]),
+    "mtip_match_partial_ex": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64,
+                                     _i64, _vp, _i32, _i32, _i32, _vp, _c.c_size_t, _vp]),
     "mtip_topk_merge": (_i32, [_i32, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp]),
     "mtip_match_topk_static": (_i32, [_i32, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i64,
                                       _i64, _vp, _i32, _vp, _vp, _vp]),

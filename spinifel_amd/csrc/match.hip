@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint8_t *__restrict__ A, int
 // dealt to the 8 XCDs round robin, so each XCD gets a contiguous range of
 // tiles, walked 8 tile-rows at a time, and the workgroups running together
 // on one XCD share A and B panels in its L2.
-template <int PREC, int K>
+template <int PREC, int K, bool EDGE>
 __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A, int64_t lda,
                                                   int64_t NA, const uint8_t *__restrict__ B,
                                                   int64_t ldb, int64_t NB, int64_t kbytes,
@@ -316,7 +316,9 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
                                                   const float *__restrict__ scaleB,
                                                   const double *__restrict__ sumsqA,
                                                   const double *__restrict__ alpha, int nta,
-                                                  int ntb, uint64_t *__restrict__ part)
+                                                  int ntb, uint64_t *__restrict__ part, int nwa,
+                                                  int nwb, int splits, float *__restrict__ spart,
+                                                  int slot)
 {
     __shared__ __attribute__((aligned(16))) uint8_t smem[2 * (BM2 + BN2) * LDS_STRIDE];
     __shared__ float s_rowc[BM2];
@@ -326,8 +328,20 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int half = lane >> 5, l32 = lane & 31;
-    int ta, tb;
-    tile_of(blockIdx.x, nta * ntb, nta, ntb, ta, tb);
+    // EDGE = false: the nwa x nwb whole tiles (nwa = nta, nwb = ntb unless
+    // the thin edge tiles go to the EDGE launch; the key layout keeps nta
+    // tile rows).  EDGE = true: each thin edge tile (last row of orientation
+    // tiles, then last column of image tiles) as `splits` workgroups over
+    // consecutive parts of K that write their raw sums to spart
+    int ta, tb, kp = 0;
+    if constexpr (!EDGE) {
+        tile_of(blockIdx.x, nwa * nwb, nwa, nwb, ta, tb);
+    } else {
+        const int t = (int)blockIdx.x / splits, n_row = nwa < nta ? ntb : 0;
+        kp = (int)blockIdx.x - t * splits;
+        ta = t < n_row ? nta - 1 : t - n_row;
+        tb = t < n_row ? t : ntb - 1;
+    }
     const int64_t a0 = (int64_t)ta * BM2, b0 = (int64_t)tb * BN2;
 
     if (tid < BM2) {
@@ -364,9 +378,19 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
         (void *)(B + b0 * ldb), (short)0, (int)((int64_t)rowsB * ldb), 0x00020000);
     const int colb = (tid & 7) * 16;
     const int vA = (tid >> 3) * (int)lda + colb, vB = (tid >> 3) * (int)ldb + colb;
+    // stages [0, nstage) from byte kb0; loads at or past kend read zeros
+    int nstage = (int)((kbytes + BKB - 1) / BKB), kb0 = 0;
+    int64_t kend = kbytes;
+    if constexpr (EDGE) {
+        const int s0 = (int)((int64_t)kp * nstage / splits);
+        const int s1 = (int)((int64_t)(kp + 1) * nstage / splits);
+        kb0 = s0 * BKB;
+        kend = min(kbytes, (int64_t)s1 * BKB);
+        nstage = s1 - s0;
+    }
     u32x4 ra[4], rb[4];
     auto load_stage = [&](int64_t kb) __attribute__((always_inline)) {
-        const bool in_k = kb + colb < kbytes;
+        const bool in_k = kb + colb < kend;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int oa = in_k ? vA + 64 * i * (int)lda + (int)kb : 0x7fffffff;
@@ -387,10 +411,9 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
         }
     };
 
-    const int nstage = (int)((kbytes + BKB - 1) / BKB);
-    load_stage(0);
+    load_stage(kb0);
     store_stage(0);
-    if (nstage > 1) load_stage(BKB);
+    if (nstage > 1) load_stage(kb0 + BKB);
     __syncthreads();
     for (int st = 0; st < nstage; ++st) {
         // stage st + 1 (fetched during stage st - 1) goes to the other buffer,
@@ -399,7 +422,7 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
         // (past the last stage these write zeros to the idle buffer and fetch
         // nothing: the loop body stays one straight-line block)
         store_stage((st + 1) & 1);
-        load_stage((int64_t)(st + 2) * BKB);
+        load_stage((int64_t)kb0 + (int64_t)(st + 2) * BKB);
         const uint8_t *sA = smem + (st & 1) * STAGE;
         const uint8_t *sB = sA + BM2 * LDS_STRIDE;
         const uint8_t *ar[4], *br[2];
@@ -492,6 +515,22 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
         __syncthreads();
     }
 
+    if constexpr (EDGE) {
+        // this part of K's raw sums over the tile's valid rows x columns
+        float *dst = spart + (int64_t)blockIdx.x * slot;
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) {
+            const int n = wn * 64 + bn * 32 + l32;
+#pragma unroll
+            for (int bm = 0; bm < 4; ++bm)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int m = wm * 128 + bm * 32 + 8 * (v >> 2) + 4 * half + (v & 3);
+                    if (m < rowsA && n < rowsB) dst[m * rowsB + n] = acc[bm][bn][v];
+                }
+        }
+        return;
+    }
     // epilogue: per image column the K smallest v over the tile's 256 rows
     const float two_a = 2.0f * (float)(*alpha);
     uint64_t top[2][K];
@@ -543,6 +582,57 @@ __global__ __launch_bounds__(512) void k_gemm_big(const uint8_t *__restrict__ A,
                 for (int r = 0; r < K; ++r) dst[r] = top[bn][r];
             }
         }
+    }
+}
+
+// Thin edge tiles computed as `splits` parts of K: per element the parts
+// summed in order, then the whole-tile epilogue's value and the per-column
+// top-K.  One workgroup per tile; a column's rows are shared by tpc
+// consecutive lanes (256 / columns, a power of two up to 16), whose lists
+// are then merged by xor shuffles.
+template <int PREC, int K>
+__global__ __launch_bounds__(256) void k_edge_fixup(const float *__restrict__ spart, int splits,
+                                                    int slot, int nta, int ntb, int nwa,
+                                                    int64_t NA, int64_t NB,
+                                                    const float *__restrict__ scaleA,
+                                                    const float *__restrict__ scaleB,
+                                                    const double *__restrict__ sumsqA,
+                                                    const double *__restrict__ alpha,
+                                                    uint64_t *__restrict__ part)
+{
+    const int t = blockIdx.x, n_row = nwa < nta ? ntb : 0;
+    const int ta = t < n_row ? nta - 1 : t - n_row, tb = t < n_row ? t : ntb - 1;
+    const int64_t a0 = (int64_t)ta * BM2, b0 = (int64_t)tb * BN2;
+    const int rA = (int)min((int64_t)BM2, NA - a0), rB = (int)min((int64_t)BN2, NB - b0);
+    int tpc = 1;
+    while (tpc < 16 && tpc * 2 * rB <= 256) tpc *= 2;
+    const int n = threadIdx.x / tpc, g = threadIdx.x % tpc;
+    const double al = *alpha;
+    const float two_a = 2.0f * (float)al;
+    uint64_t top[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) top[r] = KEY_NONE;
+    if (n < rB) {
+        const float sb = PREC == MTIP_FP8 ? scaleB[b0 + n] : 1.0f;
+        const float *src = spart + (int64_t)t * splits * slot + n;
+        for (int m = g; m < rA; m += tpc) {
+            float c = src[m * rB];
+            for (int p = 1; p < splits; ++p) c += src[(int64_t)p * slot + m * rB];
+            if (PREC == MTIP_FP8) c *= scaleA[a0 + m] * sb;
+            const float rowc = (float)(al * al * sumsqA[a0 + m]);
+            mtip::topk_insert<K>(top, mtip::make_key(fmaf(-two_a, c, rowc), (uint32_t)(a0 + m)));
+        }
+    }
+    for (int d = 1; d < tpc; d <<= 1) {
+        uint64_t other[K];
+#pragma unroll
+        for (int r = 0; r < K; ++r) other[r] = mtip::shfl_xor_u64(top[r], d);
+        mtip::topk_merge<K>(top, other);
+    }
+    if (n < rB && g == 0) {
+        uint64_t *dst = part + ((b0 + n) * nta + ta) * K;
+#pragma unroll
+        for (int r = 0; r < K; ++r) dst[r] = top[r];
     }
 }
 
@@ -989,17 +1079,69 @@ int tile_rows(int precision, int64_t N_o, int64_t N_i)
     return (precision != MTIP_FP32 && tiles >= 192) ? BM2 : BM;
 }
 
+// Thin edge tiles of the 256-row kernel: the last row of orientation tiles
+// or the last column of image tiles when it holds at most kEdgeRows valid
+// rows.  A thin tile costs as long as a whole one (its time is the K loop's
+// staging, not its MFMAs), so at 10,000 x 10,000 the 79 thin tiles of 1,600
+// add a seventh, quarter-full round of tiles on 256 CUs.  Split along K, they
+// run as one short extra launch after the 1,521 whole tiles.
+constexpr int kEdgeRows = 64;
+
+struct EdgePlan {
+    int nta, ntb, nwa, nwb, n_thin, splits, slot;
+};
+
+EdgePlan edge_plan(int64_t N_o, int64_t N_i)
+{
+    EdgePlan p;
+    p.nta = (int)mtip::ceil_div(N_o, BM2);
+    p.ntb = (int)mtip::ceil_div(N_i, BN2);
+    const int ea = (int)(N_o - (int64_t)(p.nta - 1) * BM2);
+    const int eb = (int)(N_i - (int64_t)(p.ntb - 1) * BN2);
+    const bool thin_a = p.nta > 1 && ea <= kEdgeRows, thin_b = p.ntb > 1 && eb <= kEdgeRows;
+    p.nwa = thin_a ? p.nta - 1 : p.nta;
+    p.nwb = thin_b ? p.ntb - 1 : p.ntb;
+    p.n_thin = (thin_a ? p.ntb : 0) + (thin_b ? p.nwa : 0);
+    // about one round of workgroups over the CUs, each a part of K
+    p.splits = p.n_thin ? std::max(1, std::min(8, 256 / p.n_thin)) : 1;
+    p.slot = std::max(thin_a ? ea * BN2 : 0, thin_b ? BM2 * eb : 0);
+    return p;
+}
+
+// Bytes of FP32 partial sums the split edge tiles need (0: none to split).
+size_t edge_bytes(int64_t N_o, int64_t N_i)
+{
+    if (N_o <= 0 || N_i <= 0) return 0;
+    const EdgePlan p = edge_plan(N_o, N_i);
+    return (size_t)p.n_thin * p.splits * p.slot * sizeof(float);
+}
+
 template <int PREC, int K>
 void launch_gemm_topk(const void *S, int64_t lds_b, int64_t N_o, const void *D, int64_t ldd_b,
                       int64_t N_i, int64_t kbytes, const float *ssc, const float *dsc,
                       const double *s2, const double *alpha, int tile, uint64_t *part,
-                      hipStream_t st)
+                      float *edge_ws, hipStream_t st)
 {
     if (tile == BM2) {
-        const int nta = (int)mtip::ceil_div(N_o, BM2), ntb = (int)mtip::ceil_div(N_i, BN2);
-        hipLaunchKernelGGL((k_gemm_big<PREC, K>), dim3((unsigned)(nta * ntb)), dim3(512), 0, st,
-                           (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D, ldd_b, N_i, kbytes,
-                           ssc, dsc, s2, alpha, nta, ntb, part);
+        EdgePlan p = edge_plan(N_o, N_i);
+        if (!edge_ws || !p.n_thin) {
+            p.nwa = p.nta;
+            p.nwb = p.ntb;
+            p.n_thin = 0;
+        }
+        hipLaunchKernelGGL((k_gemm_big<PREC, K, false>), dim3((unsigned)(p.nwa * p.nwb)),
+                           dim3(512), 0, st, (const uint8_t *)S, lds_b, N_o, (const uint8_t *)D,
+                           ldd_b, N_i, kbytes, ssc, dsc, s2, alpha, p.nta, p.ntb, part, p.nwa,
+                           p.nwb, 1, nullptr, 0);
+        if (p.n_thin) {
+            hipLaunchKernelGGL((k_gemm_big<PREC, K, true>), dim3((unsigned)(p.n_thin * p.splits)),
+                               dim3(512), 0, st, (const uint8_t *)S, lds_b, N_o,
+                               (const uint8_t *)D, ldd_b, N_i, kbytes, ssc, dsc, s2, alpha, p.nta,
+                               p.ntb, part, p.nwa, p.nwb, p.splits, edge_ws, p.slot);
+            hipLaunchKernelGGL((k_edge_fixup<PREC, K>), dim3((unsigned)p.n_thin), dim3(256), 0, st,
+                               edge_ws, p.splits, p.slot, p.nta, p.ntb, p.nwa, N_o, N_i, ssc, dsc,
+                               s2, alpha, part);
+        }
     } else {
         const int nta = (int)mtip::ceil_div(N_o, BM), ntb = (int)mtip::ceil_div(N_i, BN);
         hipLaunchKernelGGL((k_gemm<PREC, K, OUT_TOPK>), dim3((unsigned)(nta * ntb)), dim3(256), 0,
@@ -1195,7 +1337,7 @@ int partial_impl(const char *fn, int precision, const void *slices, int64_t ld_s
                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
                  const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
                  int64_t N_p, const double *alpha, int K, int tile, uint64_t *part,
-                 hipStream_t st)
+                 hipStream_t st, float *edge_ws)
 {
     const std::string f(fn);
     const int eb = prec_bytes(precision);
@@ -1208,16 +1350,16 @@ int partial_impl(const char *fn, int precision, const void *slices, int64_t ld_s
 #define MTIP_GEMM_CASE(P)                                                                        \
     switch (K) {                                                                                 \
     case 1: launch_gemm_topk<P, 1>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, edge_ws, st); \
             break;                                                                               \
     case 2: launch_gemm_topk<P, 2>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, edge_ws, st); \
             break;                                                                               \
     case 4: launch_gemm_topk<P, 4>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,     \
-                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
+                                   slice_scale, image_scale, slice_sumsq, alpha, tile, part, edge_ws, st); \
             break;                                                                               \
     default: launch_gemm_topk<P, 8>(slices, ld_s * eb, N_o, images, ld_d * eb, N_i, kbytes,    \
-                                    slice_scale, image_scale, slice_sumsq, alpha, tile, part, st); \
+                                    slice_scale, image_scale, slice_sumsq, alpha, tile, part, edge_ws, st); \
              break;                                                                              \
     }
     if (precision == MTIP_FP32) { MTIP_GEMM_CASE(MTIP_FP32) }
@@ -1271,6 +1413,41 @@ using mtip::partial_bytes;
 using mtip::partial_impl;
 using mtip::resolve_tile;
 
+namespace {
+
+int match_partial(const char *fn, int precision, const void *slices, int64_t ld_s,
+                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
+                  const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
+                  int64_t N_p, const double *alpha, int K, int tile_rows_arg, int flags,
+                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    const std::string f(fn);
+    int rc = check_match_args(fn, precision, slices, ld_s, slice_scale, slice_sumsq, N_o, images,
+                              ld_d, image_scale, N_i, N_p, alpha, K);
+    if (rc == MTIP_OK) rc = check_tile(fn, tile_rows_arg);
+    if (rc != MTIP_OK) return rc;
+    MTIP_REQUIRE((flags & ~MTIP_MATCH_SPLIT_EDGES) == 0, MTIP_ERR_ARG, f + ": unknown flags");
+    if (N_i == 0) return MTIP_OK;
+    const int tile = resolve_tile(precision, N_o, N_i, tile_rows_arg);
+    size_t need = partial_bytes(N_o, N_i, K, tile);
+    float *edge_ws = nullptr;
+    if ((flags & MTIP_MATCH_SPLIT_EDGES) && tile == BM2) {
+        // split only where the partial sums fit the size mtip_match_workspace_size
+        // gives, so that the choice depends on the sizes alone
+        const size_t keys = (need + 255) & ~(size_t)255, extra = edge_bytes(N_o, N_i);
+        if (extra && keys + extra <= mtip_match_workspace_size(N_o, N_i, K, precision)) {
+            need = keys + extra;
+            edge_ws = (float *)((char *)workspace + keys);
+        }
+    }
+    MTIP_REQUIRE(workspace && workspace_bytes >= need, MTIP_ERR_NOMEM, f + ": workspace too small");
+    return partial_impl(fn, precision, slices, ld_s, slice_scale, slice_sumsq, N_o, images, ld_d,
+                        image_scale, N_i, N_p, alpha, K, tile, (uint64_t *)workspace,
+                        (hipStream_t)stream, edge_ws);
+}
+
+}  // namespace
+
 extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_s,
                                   const float *slice_scale, const double *slice_sumsq, int64_t N_o,
                                   const void *images, int64_t ld_d, const float *image_scale,
@@ -1278,16 +1455,21 @@ extern "C" int mtip_match_partial(int precision, const void *slices, int64_t ld_
                                   int tile_rows_arg, void *workspace, size_t workspace_bytes,
                                   void *stream)
 {
-    int rc = check_match_args("mtip_match_partial", precision, slices, ld_s, slice_scale,
-                              slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K);
-    if (rc == MTIP_OK) rc = check_tile("mtip_match_partial", tile_rows_arg);
-    if (rc != MTIP_OK || N_i == 0) return rc;
-    const int tile = resolve_tile(precision, N_o, N_i, tile_rows_arg);
-    MTIP_REQUIRE(workspace && workspace_bytes >= partial_bytes(N_o, N_i, K, tile), MTIP_ERR_NOMEM,
-                 "mtip_match_partial: workspace too small");
-    return partial_impl("mtip_match_partial", precision, slices, ld_s, slice_scale, slice_sumsq,
-                        N_o, images, ld_d, image_scale, N_i, N_p, alpha, K, tile,
-                        (uint64_t *)workspace, (hipStream_t)stream);
+    return match_partial("mtip_match_partial", precision, slices, ld_s, slice_scale, slice_sumsq,
+                         N_o, images, ld_d, image_scale, N_i, N_p, alpha, K, tile_rows_arg, 0,
+                         workspace, workspace_bytes, stream);
+}
+
+extern "C" int mtip_match_partial_ex(int precision, const void *slices, int64_t ld_s,
+                                     const float *slice_scale, const double *slice_sumsq,
+                                     int64_t N_o, const void *images, int64_t ld_d,
+                                     const float *image_scale, int64_t N_i, int64_t N_p,
+                                     const double *alpha, int K, int tile_rows_arg, int flags,
+                                     void *workspace, size_t workspace_bytes, void *stream)
+{
+    return match_partial("mtip_match_partial_ex", precision, slices, ld_s, slice_scale,
+                         slice_sumsq, N_o, images, ld_d, image_scale, N_i, N_p, alpha, K,
+                         tile_rows_arg, flags, workspace, workspace_bytes, stream);
 }
 
 extern "C" int mtip_topk_merge(int precision, const void *workspace, int64_t N_o, int64_t N_i,

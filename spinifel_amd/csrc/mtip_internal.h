@@ -57,7 +57,7 @@ int partial_impl(const char *fn, int precision, const void *slices, int64_t ld_s
                  const float *slice_scale, const double *slice_sumsq, int64_t N_o,
                  const void *images, int64_t ld_d, const float *image_scale, int64_t N_i,
                  int64_t N_p, const double *alpha, int K, int tile, uint64_t *part,
-                 hipStream_t st);
+                 hipStream_t st, float *edge_ws = nullptr);
 int merge_impl(const uint64_t *part, int64_t N_o, int64_t N_i, int K, int tile, int32_t *top_idx,
                float *top_val, hipStream_t st);
 

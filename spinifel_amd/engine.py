@@ -661,17 +661,23 @@ class ImageSet:
                         stream=stream)
 
 
+_SPLIT_EDGES = 1      # MTIP_MATCH_SPLIT_EDGES
+
+
 def workspace_bytes(N_o, N_i, K, precision):
     return int(_lib.load().mtip_match_workspace_size(N_o, N_i, K, _lib.PRECISIONS[precision]))
 
 
 def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stream=None,
-         timers=None, tile=None):
+         timers=None, tile=None, split_edges=True):
     """Top-K orientations per image by v = alpha^2 ||s||^2 - 2 alpha s.d.
 
     S, D: Operand; returns (idx int32 (N_i, K), val float32 (N_i, K)).
     tile: GEMM tile rows (128 or 256); None takes mtip_gemm_tile's choice for
-    these sizes.
+    these sizes.  split_edges: let the 256-row kernel run thin edge tiles as
+    split-K parts (MTIP_MATCH_SPLIT_EDGES); False keeps one summation order
+    for every tile, so that an image's keys do not depend on where the batch
+    puts it.
     """
     lib = _lib.load()
     dev = D.data.device
@@ -689,11 +695,12 @@ def topk(precision, S, s_sumsq, N_o, D, N_i, N_p, alpha, K, workspace=None, stre
         tile = lib.mtip_gemm_tile(code, N_o, N_i)
         _lib.check(min(tile, 0), "mtip_gemm_tile")
     with _span(timers, "gemm_" + precision):
-        rc = lib.mtip_match_partial(code, _ptr(S.data), S.data.shape[1],
-                                    _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
-                                    D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
-                                    tile, _ptr(workspace), workspace.numel(), st)
-    _lib.check(rc, "mtip_match_partial")
+        rc = lib.mtip_match_partial_ex(code, _ptr(S.data), S.data.shape[1],
+                                       _ptr(S.scale), _ptr(s_sumsq), N_o, _ptr(D.data),
+                                       D.data.shape[1], _ptr(D.scale), N_i, N_p, _ptr(alpha), K,
+                                       tile, _SPLIT_EDGES if split_edges else 0,
+                                       _ptr(workspace), workspace.numel(), st)
+    _lib.check(rc, "mtip_match_partial_ex")
     with _span(timers, "merge"):
         rc = lib.mtip_topk_merge(code, _ptr(workspace), N_o, N_i, K, tile,
                                  _ptr(idx), _ptr(val), st)

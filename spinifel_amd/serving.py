@@ -247,8 +247,11 @@ class Server:
                                                  m.slices.row_sum, m.slices.row_sumsq, m.N_p, stream)
             D = imgs.operand(self.precision, alpha, m.slices.center, stream)
             one = engine._one(self.device)
+            # one tile height per model and no split edge tiles: an image's
+            # keys do not depend on the batch it shares
             idx, val = engine.topk(self.precision, m.operand, m.operand.sumsq, m.slices.N_o, D,
-                                   imgs.N_i, imgs.N_p, one, self.K, None, stream, tile=m.tile)
+                                   imgs.N_i, imgs.N_p, one, self.K, None, stream, tile=m.tile,
+                                   split_edges=False)
             rs = engine.rescore(m.slices.f32, imgs.f32, imgs.N_p, alpha, idx, stream)
             best = rs["best"].to("cpu", non_blocking=True)
             bd = rs["best_d64"].to("cpu", non_blocking=True)

@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
 
 def test_abi_version_and_workspace_size():
     lib = _lib.load()
-    assert lib.mtip_abi_version() == 4
+    assert lib.mtip_abi_version() == 5
     assert lib.mtip_match_workspace_size(10000, 10000, 8, 1) == 10000 * 79 * 8 * 8
     assert lib.mtip_match_workspace_size(1, 1, 1, 0) == 8
     assert lib.mtip_match_workspace_size(0, 5, 1, 0) == 0
@@ -47,6 +47,16 @@ def test_argument_errors_are_reported_without_a_gpu():
     rc = lib.mtip_match_topk(2, fake, 16, nul, fake, 4, fake, 16, nul, 4, 16, fake, 1, fake, 4096,
                              fake, fake, nul)
     assert rc == _lib.MTIP_ERR_ARG and "FP8" in _lib.last_error()
+    # split edge tiles: unknown flags, and the partial sums' bytes beside the
+    # keys (10,000 x 10,000, K = 8, 256-row tile: 79 thin tiles in 3 parts of
+    # 256 x 16 floats each) are part of the workspace the call needs
+    keys = 10000 * 40 * 8 * 8
+    rc = lib.mtip_match_partial_ex(1, fake, 16384, nul, fake, 10000, fake, 16384, nul, 10000,
+                                   16384, fake, 8, 256, 2, fake, keys, nul)
+    assert rc == _lib.MTIP_ERR_ARG and "flags" in _lib.last_error()
+    rc = lib.mtip_match_partial_ex(1, fake, 16384, nul, fake, 10000, fake, 16384, nul, 10000,
+                                   16384, fake, 8, 256, 1, fake, keys + 79 * 3 * 4096 * 4 - 1, nul)
+    assert rc == _lib.MTIP_ERR_NOMEM and "workspace" in _lib.last_error()
     rc = lib.mtip_heap_sort(nul, 4, 4, 9, nul, nul)
     assert rc == _lib.MTIP_ERR_ARG
     rc = lib.mtip_euclidean_distance(fake, fake, 4, 4, 18, fake, nul)

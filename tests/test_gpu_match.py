@@ -173,6 +173,57 @@ def test_topk_tiles_agree_after_rescoring(dev, precision):
     assert overlap >= 0.99
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_split_edge_tiles_agree_with_whole_tiles(dev, precision):
+    # 4,112 = 16 x 256 + 16 orientations and images: 17 x 17 tiles of 256, of
+    # which the 33 thin edge tiles (16 valid rows or columns) run as parts of
+    # K after the 256 whole ones (MTIP_MATCH_SPLIT_EDGES; their partial sums
+    # fit beside the keys in mtip_match_workspace_size).  Keys of whole tiles
+    # are bitwise the same with and without the split, keys of edge tiles
+    # agree to FP32 rounding, and the FP64 rescore gives the oracle's argmin.
+    import torch
+    from spinifel_amd import engine
+    N_o = N_i = 4112
+    N_p = 512
+    print(f"case: split edge tiles {precision} {(N_o, N_i, N_p)}", flush=True)
+    rng = np.random.default_rng(21)
+    S = rng.gamma(2.0, 1.0, (N_o, N_p))
+    Dm = rng.poisson(S[rng.integers(0, N_o, N_i)] * 1.2).astype(np.float64)
+    ss = _sliceset(S, dev)
+    im = engine.ImageSet(Dm, dev)
+    a = torch.tensor([1.2], dtype=torch.float64, device=dev)
+    Sop = ss.operand(precision, True)
+    Dop = im.operand(precision, a, ss.center)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+    res = {}
+    for split in (False, True):
+        idx, val = engine.topk(precision, Sop, Sop.sumsq, N_o, Dop, N_i, N_p, one, 8, tile=256,
+                               split_edges=split)
+        r = engine.rescore(ss.f32, im.f32, N_p, a, idx)
+        res[split] = (idx.cpu().numpy(), val.cpu().numpy(), r["best"].cpu().numpy())
+    ref_idx, _ = matching.topk_squared(1.2 * S.astype(np.float32).astype(np.float64),
+                                       Dm.astype(np.float32).astype(np.float64), 8)
+    for split in (False, True):
+        np.testing.assert_array_equal(res[split][2], ref_idx[:, 0])
+    iw, vw, _ = res[False]
+    isp, vsp, _ = res[True]
+    scale = float(np.abs(vw).max())
+    n_whole = n_edge = 0
+    for i in range(N_i):
+        dw, dsp = dict(zip(iw[i], vw[i])), dict(zip(isp[i], vsp[i]))
+        for o in set(dw) & set(dsp):
+            if i < 4096 and o < 4096:
+                n_whole += 1
+                assert dw[o] == dsp[o], (i, o, dw[o], dsp[o])
+            else:
+                n_edge += 1
+                assert abs(dw[o] - dsp[o]) <= 1e-5 * scale, (i, o, dw[o], dsp[o])
+    overlap = np.mean([len(set(x) & set(y)) / 8 for x, y in zip(iw, isp)])
+    print(f"whole-tile keys compared {n_whole}, edge keys {n_edge}, overlap {overlap:.4f}",
+          flush=True)
+    assert n_whole > 0 and n_edge > 0 and overlap >= 0.99
+
+
 def test_ties_lowest_index(dev):
     import torch
     from spinifel_amd import engine
